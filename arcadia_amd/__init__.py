@@ -1,0 +1,31 @@
+"""arcadia_amd — MI355X-native 64-tree DDA voxel ray traversal (Arcadia's trace_world_64tree).
+
+The compute path is the HIP library libarcadia_hip.so (C ABI in include/); this package is
+its Python plumbing: ctypes bindings, world/ray helpers and device-memory glue for tests and
+benchmarks.
+"""
+from .native import (  # noqa: F401
+    ARC_GEN_CAVES,
+    ARC_GEN_HEIGHTMAP,
+    ARC_GEN_SPARSE,
+    ARC_SEM_CHUNK,
+    ARC_SEM_WORLD,
+    BRICK_DTYPE,
+    HIT_DTYPE,
+    NODE_DTYPE,
+    RAY_DTYPE,
+    RESULT_DTYPE,
+    ArcError,
+    Context,
+    HostWorld,
+    TraceParams,
+    bounce_rays,
+    camera_look,
+    gen_params,
+    lib,
+    lod_payload,
+    palette_material,
+    primary_rays,
+)
+
+__all__ = [n for n in dir() if not n.startswith("_")]

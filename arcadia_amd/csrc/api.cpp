@@ -1,0 +1,365 @@
+// C ABI of the traversal library: context, device world pools, trace entry points.
+// Host code only (built with hipcc as HIP so it can call the launchers).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/arcadia_hip.h"
+#include "trace_launch.h"
+
+struct arc_ctx {
+  int device = 0;
+  uint32_t n_cu = 0;
+  uint32_t lds_per_block = 0;
+  std::string err;
+  // device pools (the @group(0) bindings 1..3)
+  uint32_t grid[3] = {0, 0, 0};
+  uint32_t* d_roots = nullptr;
+  arc_node64* d_nodes = nullptr;
+  arc_brick64* d_bricks = nullptr;
+  uint64_t n_nodes = 0, node_cap = 0, n_bricks = 0, brick_cap = 0;
+  bool has_world = false;
+  // persistent-queue counters (2 x u32), one pair per concurrently usable stream slot
+  uint32_t* d_queue = nullptr;
+  // scratch for the host-pointer entry point
+  arc_ray* d_rays = nullptr;
+  arc_hit* d_hits = nullptr;
+  uint64_t scratch_n = 0;
+  arc_launch_config cfg{};
+};
+
+namespace {
+
+int fail(arc_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                   \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail((ctx), ARC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+arc_launch_config default_config() {
+  arc_launch_config c;
+  c.blocks_per_cu = 8;
+  c.block_threads = 256;
+  c.refill_lanes = 16;
+  c.batch_rays = 64;
+  return c;
+}
+
+// Structural check of uploaded pools: every reachable node's children lie inside the node
+// pool, mixed leaves point inside the brick pool, and trees are at most four levels deep with
+// leaves at the last level. Malformed pools are refused before any kernel reads them.
+int check_tree(const arc_node64* nodes, uint64_t n_nodes, uint64_t n_bricks, uint32_t root, std::string& why) {
+  struct Item {
+    uint32_t idx, level;
+  };
+  std::vector<Item> stack;
+  stack.push_back({root, 0});
+  while (!stack.empty()) {
+    Item it = stack.back();
+    stack.pop_back();
+    if (it.idx >= n_nodes) {
+      why = "node index " + std::to_string(it.idx) + " outside the node pool";
+      return ARC_ERR_INVALID;
+    }
+    const arc_node64& n = nodes[it.idx];
+    const uint64_t mask = (uint64_t)n.mask_low | ((uint64_t)n.mask_high << 32);
+    if (mask == 0) continue;
+    const uint32_t base = n.child_ptr & ARC_PTR_INDEX_MASK;
+    if (n.child_ptr & ARC_IS_LEAF_FLAG) {
+      if (!(n.child_ptr & ARC_IS_UNIFORM_FLAG) && base >= n_bricks) {
+        why = "brick index " + std::to_string(base) + " outside the brick pool";
+        return ARC_ERR_INVALID;
+      }
+      continue;
+    }
+    if (it.level >= 3) {
+      why = "internal node " + std::to_string(it.idx) + " below level 3";
+      return ARC_ERR_INVALID;
+    }
+    const uint32_t cnt = (uint32_t)__builtin_popcountll(mask);
+    if ((uint64_t)base + cnt > n_nodes) {
+      why = "children of node " + std::to_string(it.idx) + " outside the node pool";
+      return ARC_ERR_INVALID;
+    }
+    for (uint32_t k = 0; k < cnt; ++k) stack.push_back({base + k, it.level + 1});
+  }
+  return ARC_OK;
+}
+
+bool valid_params(const arc_trace_params* p) {
+  if (!p) return false;
+  if (p->semantics != ARC_SEM_WORLD && p->semantics != ARC_SEM_CHUNK) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int arc_create(int device, arc_ctx** out) {
+  if (!out) return ARC_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return ARC_ERR_INVALID;
+  arc_ctx* c = new (std::nothrow) arc_ctx();
+  if (!c) return ARC_ERR_INVALID;
+  c->device = device;
+  c->cfg = default_config();
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return ARC_ERR_HIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return ARC_ERR_HIP;
+  }
+  c->n_cu = (uint32_t)prop.multiProcessorCount;
+  c->lds_per_block = (uint32_t)prop.sharedMemPerBlock;
+  if (hipMalloc(&c->d_queue, 64 * sizeof(uint32_t)) != hipSuccess) {
+    delete c;
+    return ARC_ERR_HIP;
+  }
+  if (hipMemset(c->d_queue, 0, 64 * sizeof(uint32_t)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    hipFree(c->d_queue);
+    delete c;
+    return ARC_ERR_HIP;
+  }
+  *out = c;
+  return ARC_OK;
+}
+
+void arc_destroy(arc_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  hipFree(c->d_roots);
+  hipFree(c->d_nodes);
+  hipFree(c->d_bricks);
+  hipFree(c->d_queue);
+  hipFree(c->d_rays);
+  hipFree(c->d_hits);
+  delete c;
+}
+
+const char* arc_last_error(const arc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int arc_device_info(const arc_ctx* c, uint32_t* n_cu, uint32_t* lds) {
+  if (!c) return ARC_ERR_INVALID;
+  if (n_cu) *n_cu = c->n_cu;
+  if (lds) *lds = c->lds_per_block;
+  return ARC_OK;
+}
+
+int arc_set_launch_config(arc_ctx* c, const arc_launch_config* cfg) {
+  if (!c || !cfg) return ARC_ERR_INVALID;
+  arc_launch_config d = default_config();
+  arc_launch_config n = *cfg;
+  if (n.blocks_per_cu == 0) n.blocks_per_cu = d.blocks_per_cu;
+  if (n.block_threads == 0) n.block_threads = d.block_threads;
+  if (n.refill_lanes == 0) n.refill_lanes = d.refill_lanes;
+  if (n.batch_rays == 0) n.batch_rays = d.batch_rays;
+  if (n.block_threads % 64 != 0 || n.block_threads > 256) return fail(c, ARC_ERR_INVALID, "block_threads must be 64..256, a multiple of 64");
+  if (n.refill_lanes > 64) return fail(c, ARC_ERR_INVALID, "refill_lanes must be 1..64");
+  if (n.blocks_per_cu > 64) return fail(c, ARC_ERR_INVALID, "blocks_per_cu must be 1..64");
+  c->cfg = n;
+  return ARC_OK;
+}
+
+int arc_get_launch_config(const arc_ctx* c, arc_launch_config* cfg) {
+  if (!c || !cfg) return ARC_ERR_INVALID;
+  *cfg = c->cfg;
+  return ARC_OK;
+}
+
+int arc_world_upload(arc_ctx* c, const uint32_t grid[3], const uint32_t* roots, const arc_node64* nodes,
+                     uint64_t n_nodes, uint64_t node_cap, const arc_brick64* bricks, uint64_t n_bricks,
+                     uint64_t brick_cap) {
+  if (!c || !grid || !roots || !nodes || !bricks || n_nodes == 0 || n_bricks == 0)
+    return fail(c, ARC_ERR_INVALID, "arc_world_upload: null pointer or empty pool");
+  for (int a = 0; a < 3; ++a)
+    if (grid[a] == 0 || grid[a] > 256) return fail(c, ARC_ERR_INVALID, "grid dims must be 1..256");
+  if (n_nodes > ARC_PTR_INDEX_MASK || n_bricks > ARC_PTR_INDEX_MASK)
+    return fail(c, ARC_ERR_INVALID, "pool larger than the 30-bit pointer range");
+  if (node_cap < n_nodes) node_cap = n_nodes;
+  if (brick_cap < n_bricks) brick_cap = n_bricks;
+  const uint64_t nchunks = (uint64_t)grid[0] * grid[1] * grid[2];
+  for (uint64_t i = 0; i < nchunks; ++i) {
+    if (roots[i] == ARC_EMPTY_CHUNK_SENTINEL) continue;
+    std::string why;
+    if (check_tree(nodes, n_nodes, n_bricks, roots[i], why) != ARC_OK)
+      return fail(c, ARC_ERR_INVALID, "chunk %llu: %s", (unsigned long long)i, why.c_str());
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipDeviceSynchronize());
+  hipFree(c->d_roots);
+  hipFree(c->d_nodes);
+  hipFree(c->d_bricks);
+  c->d_roots = nullptr;
+  c->d_nodes = nullptr;
+  c->d_bricks = nullptr;
+  c->has_world = false;
+  HIP_TRY(c, hipMalloc(&c->d_roots, nchunks * sizeof(uint32_t)));
+  HIP_TRY(c, hipMalloc(&c->d_nodes, node_cap * sizeof(arc_node64)));
+  HIP_TRY(c, hipMalloc(&c->d_bricks, brick_cap * sizeof(arc_brick64)));
+  HIP_TRY(c, hipMemcpy(c->d_roots, roots, nchunks * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_nodes, nodes, n_nodes * sizeof(arc_node64), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_bricks, bricks, n_bricks * sizeof(arc_brick64), hipMemcpyHostToDevice));
+  if (node_cap > n_nodes) HIP_TRY(c, hipMemset(c->d_nodes + n_nodes, 0, (node_cap - n_nodes) * sizeof(arc_node64)));
+  if (brick_cap > n_bricks)
+    HIP_TRY(c, hipMemset(c->d_bricks + n_bricks, 0, (brick_cap - n_bricks) * sizeof(arc_brick64)));
+  HIP_TRY(c, hipDeviceSynchronize());
+  std::memcpy(c->grid, grid, sizeof c->grid);
+  c->n_nodes = n_nodes;
+  c->node_cap = node_cap;
+  c->n_bricks = n_bricks;
+  c->brick_cap = brick_cap;
+  c->has_world = true;
+  return ARC_OK;
+}
+
+int arc_world_device_pools(const arc_ctx* c, const uint32_t** roots, const arc_node64** nodes,
+                           const arc_brick64** bricks) {
+  if (!c || !c->has_world) return ARC_ERR_NO_WORLD;
+  if (roots) *roots = c->d_roots;
+  if (nodes) *nodes = c->d_nodes;
+  if (bricks) *bricks = c->d_bricks;
+  return ARC_OK;
+}
+
+int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "arc_world_patch before arc_world_upload");
+  if (n && !patches) return fail(c, ARC_ERR_INVALID, "null patches");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const uint64_t nchunks = (uint64_t)c->grid[0] * c->grid[1] * c->grid[2];
+  // Grow the pools first (renderer.rs:156-223: copy old contents into a larger buffer).
+  uint64_t need_nodes = c->node_cap, need_bricks = c->brick_cap;
+  for (uint64_t i = 0; i < n; ++i) {
+    const arc_gpu_patch& p = patches[i];
+    if (p.chunk_flat_idx >= nchunks) return fail(c, ARC_ERR_INVALID, "patch %llu: chunk index out of range", (unsigned long long)i);
+    if ((p.n_nodes && !p.nodes) || (p.n_bricks && !p.bricks)) return fail(c, ARC_ERR_INVALID, "patch %llu: null slice", (unsigned long long)i);
+    uint64_t ne = (uint64_t)p.node_offset + p.n_nodes, be = (uint64_t)p.brick_offset + p.n_bricks;
+    if (ne > ARC_PTR_INDEX_MASK || be > ARC_PTR_INDEX_MASK) return fail(c, ARC_ERR_CAPACITY, "patch %llu beyond the 30-bit pointer range", (unsigned long long)i);
+    if (ne > need_nodes) need_nodes = ne;
+    if (be > need_bricks) need_bricks = be;
+  }
+  if (need_nodes > c->node_cap) {
+    uint64_t cap = need_nodes + (need_nodes >> 2) + 1024;
+    arc_node64* nn = nullptr;
+    HIP_TRY(c, hipMalloc(&nn, cap * sizeof(arc_node64)));
+    HIP_TRY(c, hipMemcpy(nn, c->d_nodes, c->node_cap * sizeof(arc_node64), hipMemcpyDeviceToDevice));
+    HIP_TRY(c, hipMemset(nn + c->node_cap, 0, (cap - c->node_cap) * sizeof(arc_node64)));
+    HIP_TRY(c, hipDeviceSynchronize());
+    hipFree(c->d_nodes);
+    c->d_nodes = nn;
+    c->node_cap = cap;
+  }
+  if (need_bricks > c->brick_cap) {
+    uint64_t cap = need_bricks + (need_bricks >> 2) + 1024;
+    arc_brick64* nb = nullptr;
+    HIP_TRY(c, hipMalloc(&nb, cap * sizeof(arc_brick64)));
+    HIP_TRY(c, hipMemcpy(nb, c->d_bricks, c->brick_cap * sizeof(arc_brick64), hipMemcpyDeviceToDevice));
+    HIP_TRY(c, hipMemset(nb + c->brick_cap, 0, (cap - c->brick_cap) * sizeof(arc_brick64)));
+    HIP_TRY(c, hipDeviceSynchronize());
+    hipFree(c->d_bricks);
+    c->d_bricks = nb;
+    c->brick_cap = cap;
+  }
+  for (uint64_t i = 0; i < n; ++i) {
+    const arc_gpu_patch& p = patches[i];
+    if (p.n_nodes)
+      HIP_TRY(c, hipMemcpy(c->d_nodes + p.node_offset, p.nodes, (size_t)p.n_nodes * sizeof(arc_node64), hipMemcpyHostToDevice));
+    if (p.n_bricks)
+      HIP_TRY(c, hipMemcpy(c->d_bricks + p.brick_offset, p.bricks, (size_t)p.n_bricks * sizeof(arc_brick64), hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->d_roots + p.chunk_flat_idx, &p.new_root_node_idx, sizeof(uint32_t), hipMemcpyHostToDevice));
+    uint64_t ne = (uint64_t)p.node_offset + p.n_nodes, be = (uint64_t)p.brick_offset + p.n_bricks;
+    if (ne > c->n_nodes) c->n_nodes = ne;
+    if (be > c->n_bricks) c->n_bricks = be;
+  }
+  HIP_TRY(c, hipDeviceSynchronize());
+  return ARC_OK;
+}
+
+static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n, void* d_out, bool full,
+                  void* stream) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!valid_params(params)) return fail(c, ARC_ERR_INVALID, "invalid trace params");
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "trace before arc_world_upload");
+  if (n == 0) return ARC_OK;
+  if (!d_rays || !d_out) return fail(c, ARC_ERR_INVALID, "null ray or output pointer");
+  if (n > 0x7FFFFFFFull) return fail(c, ARC_ERR_INVALID, "at most 2^31-1 rays per launch");
+  arcdev::LaunchArgs a{};
+  std::memcpy(a.grid, c->grid, sizeof a.grid);
+  a.roots = c->d_roots;
+  a.nodes = c->d_nodes;
+  a.bricks = c->d_bricks;
+  a.params = *params;
+  a.rays = d_rays;
+  a.out = d_out;
+  a.n = n;
+  a.full = full;
+  a.queue = c->d_queue;
+  uint64_t want_blocks = (uint64_t)c->cfg.blocks_per_cu * c->n_cu;
+  uint64_t max_useful = (n + c->cfg.block_threads - 1) / c->cfg.block_threads;
+  a.blocks = (uint32_t)(want_blocks < max_useful ? want_blocks : max_useful);
+  if (a.blocks == 0) a.blocks = 1;
+  a.block_threads = c->cfg.block_threads;
+  a.refill = c->cfg.refill_lanes;
+  a.batch = c->cfg.batch_rays;
+  hipError_t e = arcdev::launch_trace(a, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
+int arc_trace_device(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n, arc_hit* d_hits,
+                     void* stream) {
+  return launch(c, params, d_rays, n, d_hits, false, stream);
+}
+
+int arc_trace_full_device(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n,
+                          arc_trace_result* d_out, void* stream) {
+  return launch(c, params, d_rays, n, d_out, true, stream);
+}
+
+int arc_trace(arc_ctx* c, const arc_trace_params* params, const arc_ray* rays, uint64_t n, arc_hit* hits_out) {
+  if (!c) return ARC_ERR_INVALID;
+  if (n == 0) return ARC_OK;
+  if (!rays || !hits_out) return fail(c, ARC_ERR_INVALID, "null ray or hit pointer");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (n > c->scratch_n) {
+    hipFree(c->d_rays);
+    hipFree(c->d_hits);
+    c->d_rays = nullptr;
+    c->d_hits = nullptr;
+    c->scratch_n = 0;
+    HIP_TRY(c, hipMalloc(&c->d_rays, n * sizeof(arc_ray)));
+    HIP_TRY(c, hipMalloc(&c->d_hits, n * sizeof(arc_hit)));
+    c->scratch_n = n;
+  }
+  HIP_TRY(c, hipMemcpy(c->d_rays, rays, n * sizeof(arc_ray), hipMemcpyHostToDevice));
+  int rc = launch(c, params, c->d_rays, n, c->d_hits, false, nullptr);
+  if (rc != ARC_OK) return rc;
+  HIP_TRY(c, hipDeviceSynchronize());
+  HIP_TRY(c, hipMemcpy(hits_out, c->d_hits, n * sizeof(arc_hit), hipMemcpyDeviceToHost));
+  return ARC_OK;
+}
+
+}  // extern "C"

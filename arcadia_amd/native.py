@@ -1,0 +1,411 @@
+"""ctypes bindings of libarcadia_hip.so (C ABI: include/arcadia_hip.h, include/arcadia_world.h).
+
+The library is the product: there is no Python or CPU fallback for the traversal. Loading
+fails loudly when the shared library has not been built (python -m arcadia_amd.build).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libarcadia_hip.so"
+
+ARC_OK = 0
+ARC_ERR_INVALID = -1
+ARC_ERR_HIP = -2
+ARC_ERR_NO_WORLD = -3
+ARC_ERR_CAPACITY = -4
+
+ARC_SEM_WORLD = 0
+ARC_SEM_CHUNK = 1
+ARC_GEN_HEIGHTMAP = 0
+ARC_GEN_CAVES = 1
+ARC_GEN_SPARSE = 2
+ARC_FACE_NONE = 255
+ARC_HIT_FLAG_HIT = 1
+ARC_HIT_FLAG_LOD = 2
+EMPTY_CHUNK_SENTINEL = 0xFFFFFFFF
+
+NODE_DTYPE = np.dtype([("mask_low", "<u4"), ("mask_high", "<u4"), ("child_ptr", "<u4"), ("lod_payload", "<u4")])
+BRICK_DTYPE = np.dtype([("packed_materials", "<u4", (16,))])
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("t_start", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("t_max", "<f4")])
+HIT_DTYPE = np.dtype([("t_voxels", "<f4"), ("vx", "<u2"), ("vy", "<u2"), ("vz", "<u2"), ("mat", "u1"),
+                      ("face", "u1"), ("steps", "<u2"), ("flags", "<u2")])
+RESULT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("normal", "<f4", (3,)),
+                         ("lod_rgb", "<f4", (3,)), ("contact_ao", "<f4"), ("mat", "<u4"), ("voxel_hash", "<u4"),
+                         ("steps", "<u4"), ("hit", "<u4"), ("vx", "<u4"), ("vy", "<u4"), ("vz", "<u4"),
+                         ("reserved", "<u4", (2,))])
+assert NODE_DTYPE.itemsize == 16 and BRICK_DTYPE.itemsize == 64
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and RESULT_DTYPE.itemsize == 80
+
+
+class TraceParams(C.Structure):
+    """arc_trace_params: the traversal fields of CameraUniforms (types.wgsl:19-63)."""
+
+    _fields_ = [("semantics", C.c_uint32), ("lod_bias", C.c_uint32), ("ignore_water", C.c_uint32),
+                ("max_steps", C.c_uint32 * 3), ("lod_dist_m", C.c_float * 3), ("lod_step", C.c_uint32 * 3),
+                ("ao_max_dist_bits", C.c_float), ("reserved", C.c_uint32 * 3)]
+
+    @classmethod
+    def exact(cls, lod_bias: int = 0, ignore_water: bool = False, max_steps: int = 4096) -> "TraceParams":
+        """Full traversal: LOD off (thresholds beyond any distance/step) and a step cap no ray reaches."""
+        p = cls()
+        p.semantics = ARC_SEM_WORLD
+        p.lod_bias = lod_bias
+        p.ignore_water = int(ignore_water)
+        for i in range(3):
+            p.max_steps[i] = max_steps
+            p.lod_dist_m[i] = 1.0e30
+            p.lod_step[i] = 0xFFFFFFFF
+        p.ao_max_dist_bits = 24.0
+        return p
+
+    @classmethod
+    def reference_default(cls, lod_bias: int = 0, ignore_water: bool = False) -> "TraceParams":
+        """RaymarchQualitySettings::high_quality (crates/arcadia-render/src/quality.rs:101-134)."""
+        p = cls()
+        p.semantics = ARC_SEM_WORLD
+        p.lod_bias = lod_bias
+        p.ignore_water = int(ignore_water)
+        p.max_steps[0], p.max_steps[1], p.max_steps[2] = 200, 72, 48
+        p.lod_dist_m[0], p.lod_dist_m[1], p.lod_dist_m[2] = 120.0, 28.0, 24.0
+        p.lod_step[0], p.lod_step[1], p.lod_step[2] = 155, 60, 48
+        p.ao_max_dist_bits = 24.0
+        return p
+
+    @classmethod
+    def chunk(cls) -> "TraceParams":
+        """raycast_chunk_tree semantics (crates/arcadia-core/src/raycast_cpu.rs:28-218)."""
+        p = cls.exact()
+        p.semantics = ARC_SEM_CHUNK
+        return p
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_uint32), ("block_threads", C.c_uint32), ("refill_lanes", C.c_uint32),
+                ("batch_rays", C.c_uint32)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("octaves", C.c_uint32), ("seed", C.c_uint64), ("height_min", C.c_float),
+                ("height_max", C.c_float), ("wavelength", C.c_float), ("density_threshold", C.c_float),
+                ("n_cuboids", C.c_uint32), ("cuboid_min", C.c_uint32), ("cuboid_max", C.c_uint32),
+                ("mat_min", C.c_uint32), ("mat_max", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3),
+                ("tan_half_fov", C.c_float), ("aspect", C.c_float), ("jitter", C.c_float * 2)]
+
+
+class GpuPatch(C.Structure):
+    _fields_ = [("chunk_flat_idx", C.c_uint32), ("new_root_node_idx", C.c_uint32), ("node_offset", C.c_uint32),
+                ("n_nodes", C.c_uint32), ("nodes", C.c_void_p), ("brick_offset", C.c_uint32),
+                ("n_bricks", C.c_uint32), ("bricks", C.c_void_p)]
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_SIGS = [
+    ("arc_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("arc_destroy", None, [C.c_void_p]),
+    ("arc_last_error", C.c_char_p, [C.c_void_p]),
+    ("arc_device_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("arc_world_upload", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.c_void_p, C.c_void_p, C.c_uint64,
+                                   C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint64]),
+    ("arc_world_patch", C.c_int, [C.c_void_p, C.POINTER(GpuPatch), C.c_uint64]),
+    ("arc_world_device_pools", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                         C.POINTER(C.c_void_p)]),
+    ("arc_trace", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("arc_trace_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
+                                   C.c_void_p]),
+    ("arc_trace_full_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
+                                        C.c_void_p]),
+    ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("arc_world_host_create", C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
+    ("arc_world_host_destroy", None, [C.c_void_p]),
+    ("arc_world_host_set_chunk_dense", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("arc_world_host_set_chunk_sparse", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64]),
+    ("arc_world_host_chunk_tree", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p,
+                                            C.POINTER(C.c_uint64)]),
+    ("arc_world_host_pack", C.c_int, [C.c_void_p]),
+    ("arc_world_host_pools", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_uint64)]),
+    ("arc_palette_material", C.c_int, [C.c_uint32, C.POINTER(C.c_float)]),
+    ("arc_palette_lod_payload", C.c_uint32, [C.c_uint32]),
+    ("arc_world_generate", C.c_int, [C.c_void_p, C.POINTER(GenParams), C.c_int]),
+    ("arc_gen_voxel", C.c_uint32, [C.c_void_p, C.POINTER(GenParams), C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("arc_camera_look", C.c_int, [C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+                                  C.POINTER(Camera)]),
+    ("arc_gen_primary_rays", C.c_int, [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_void_p,
+                                       C.c_void_p]),
+    ("arc_gen_bounce_rays", C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_float, C.c_void_p,
+                                        C.c_void_p]),
+]
+
+EXPORTED = [name for name, _, _ in _SIGS]
+
+
+def lib() -> C.CDLL:
+    """Load the native library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m arcadia_amd.build`")
+        # One HIP runtime per process: torch ships its own libamdhip64 (same soname). Loading
+        # torch first lets this library bind to that copy, so device pointers and streams from
+        # torch and from the C ABI live in the same runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        h = C.CDLL(str(LIB_PATH))
+        for name, res, args in _SIGS:
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+class ArcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"arcadia error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------------------------------------
+# host world
+# ---------------------------------------------------------------------------------------------
+def gen_params(kind: int, seed: int = 1, **kw) -> GenParams:
+    """Generator parameters with the workload defaults of each world kind."""
+    p = GenParams()
+    p.kind = kind
+    p.seed = seed
+    p.octaves = 5
+    p.mat_min, p.mat_max = 1, 15
+    if kind == ARC_GEN_HEIGHTMAP:
+        p.height_min, p.height_max, p.wavelength = 32.0, 200.0, 128.0
+    elif kind == ARC_GEN_CAVES:
+        p.height_min, p.height_max, p.wavelength = 192.0, 704.0, 384.0
+        p.density_threshold = 0.45
+        p.n_cuboids, p.cuboid_min, p.cuboid_max = 2000, 2, 64
+    else:
+        p.height_min, p.height_max, p.wavelength = 32.0, 224.0, 512.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class HostWorld:
+    """CPU-side world: per-chunk 64-trees and the packed pools (WorldMap, world_map.rs:30-205)."""
+
+    def __init__(self, grid=(1, 1, 1)):
+        self.grid = tuple(int(g) for g in grid)
+        h = C.c_void_p()
+        g = (C.c_uint32 * 3)(*self.grid)
+        rc = lib().arc_world_host_create(g, C.byref(h))
+        if rc != ARC_OK:
+            raise ArcError(rc, "arc_world_host_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().arc_world_host_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def chunk_index(self, cx: int, cy: int, cz: int) -> int:
+        return cx + cy * self.grid[0] + cz * self.grid[0] * self.grid[1]
+
+    def set_chunk_dense(self, idx: int, dense: np.ndarray) -> None:
+        """dense: uint8 [256,256,256] indexed [z, y, x]."""
+        d = np.ascontiguousarray(dense, dtype=np.uint8)
+        assert d.shape == (256, 256, 256)
+        rc = lib().arc_world_host_set_chunk_dense(self._h, idx, _ptr(d))
+        if rc != ARC_OK:
+            raise ArcError(rc, "set_chunk_dense")
+
+    def set_chunk_sparse(self, idx: int, voxels) -> None:
+        """voxels: iterable of (lx, ly, lz, mat) — ChunkTree64::from_sparse_voxels."""
+        v = np.ascontiguousarray(np.asarray(voxels, dtype=np.uint32).reshape(-1, 4))
+        rc = lib().arc_world_host_set_chunk_sparse(self._h, idx, _ptr(v) if v.size else None, v.shape[0])
+        if rc != ARC_OK:
+            raise ArcError(rc, "set_chunk_sparse")
+
+    def chunk_tree(self, idx: int):
+        nn, nb = C.c_uint64(0), C.c_uint64(0)
+        rc = lib().arc_world_host_chunk_tree(self._h, idx, None, C.byref(nn), None, C.byref(nb))
+        if rc != ARC_OK:
+            raise ArcError(rc, "chunk_tree")
+        nodes = np.zeros(nn.value, NODE_DTYPE)
+        bricks = np.zeros(max(nb.value, 1), BRICK_DTYPE)
+        rc = lib().arc_world_host_chunk_tree(self._h, idx, _ptr(nodes), C.byref(nn), _ptr(bricks), C.byref(nb))
+        if rc != ARC_OK:
+            raise ArcError(rc, "chunk_tree")
+        return nodes, bricks[: nb.value]
+
+    def generate(self, params: GenParams, nthreads: int | None = None) -> None:
+        nt = nthreads or min(16, os.cpu_count() or 1)
+        rc = lib().arc_world_generate(self._h, C.byref(params), nt)
+        if rc != ARC_OK:
+            raise ArcError(rc, "arc_world_generate")
+
+    def gen_voxel(self, params: GenParams, x: int, y: int, z: int) -> int:
+        return int(lib().arc_gen_voxel(self._h, C.byref(params), x, y, z))
+
+    def pack(self) -> None:
+        rc = lib().arc_world_host_pack(self._h)
+        if rc != ARC_OK:
+            raise ArcError(rc, "pack")
+
+    def pools(self):
+        """Copies of (grid, chunk_roots, node_pool, brick_pool) after pack()."""
+        g = (C.c_uint32 * 3)()
+        r, n, b = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nn, nb = C.c_uint64(), C.c_uint64()
+        rc = lib().arc_world_host_pools(self._h, g, C.byref(r), C.byref(n), C.byref(nn), C.byref(b), C.byref(nb))
+        if rc != ARC_OK:
+            raise ArcError(rc, "pools (call pack() first)")
+        nch = g[0] * g[1] * g[2]
+        roots = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_uint32)), (nch,)).copy()
+        nodes = np.frombuffer((C.c_char * (nn.value * 16)).from_address(n.value), NODE_DTYPE).copy()
+        bricks = np.frombuffer((C.c_char * (nb.value * 64)).from_address(b.value), BRICK_DTYPE).copy()
+        return (g[0], g[1], g[2]), roots, nodes, bricks
+
+
+def palette_material(mat: int) -> np.ndarray:
+    out = (C.c_float * 8)()
+    rc = lib().arc_palette_material(mat, out)
+    if rc != ARC_OK:
+        raise ArcError(rc, "palette")
+    return np.array(out[:], np.float32)
+
+
+def lod_payload(mat: int) -> int:
+    return int(lib().arc_palette_lod_payload(mat))
+
+
+# ---------------------------------------------------------------------------------------------
+# rays
+# ---------------------------------------------------------------------------------------------
+def camera_look(eye, yaw: float, pitch: float, fov_y_deg: float, width: int, height: int) -> Camera:
+    cam = Camera()
+    e = (C.c_float * 3)(*eye)
+    rc = lib().arc_camera_look(e, yaw, pitch, fov_y_deg, width, height, C.byref(cam))
+    if rc != ARC_OK:
+        raise ArcError(rc, "camera_look")
+    return cam
+
+
+def primary_rays(cam: Camera, width: int, height: int, max_dist_m: float = 250.0):
+    """(rays, pixel_of_ray) in 8x8-tile order."""
+    rays = np.zeros(width * height, RAY_DTYPE)
+    pix = np.zeros(width * height, np.uint32)
+    rc = lib().arc_gen_primary_rays(C.byref(cam), width, height, max_dist_m, _ptr(rays), _ptr(pix))
+    if rc != ARC_OK:
+        raise ArcError(rc, "primary_rays")
+    return rays, pix
+
+
+def bounce_rays(prays: np.ndarray, hits: np.ndarray, seed: int = 1, max_dist_m: float = 26.0):
+    """(rays, src_index): one cosine bounce per hit primary ray."""
+    n = len(prays)
+    out = np.zeros(max(n, 1), RAY_DTYPE)
+    src = np.zeros(max(n, 1), np.uint32)
+    k = lib().arc_gen_bounce_rays(_ptr(prays), _ptr(hits), n, seed, max_dist_m, _ptr(out), _ptr(src))
+    if k < 0:
+        raise ArcError(int(k), "bounce_rays")
+    return out[:k].copy(), src[:k].copy()
+
+
+# ---------------------------------------------------------------------------------------------
+# device context
+# ---------------------------------------------------------------------------------------------
+class Context:
+    """A device context holding the world pools (Renderer's Group0Resources equivalent)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().arc_create(device, C.byref(h))
+        if rc != ARC_OK:
+            raise ArcError(rc, "arc_create (no usable HIP device?)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().arc_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _check(self, rc: int, what: str):
+        if rc != ARC_OK:
+            raise ArcError(rc, f"{what}: {lib().arc_last_error(self._h).decode()}")
+
+    def device_info(self):
+        cu, lds = C.c_uint32(), C.c_uint32()
+        self._check(lib().arc_device_info(self._h, C.byref(cu), C.byref(lds)), "device_info")
+        return cu.value, lds.value
+
+    def upload(self, grid, roots, nodes, bricks, node_capacity: int = 0, brick_capacity: int = 0):
+        g = (C.c_uint32 * 3)(*grid)
+        roots = np.ascontiguousarray(roots, np.uint32)
+        nodes = np.ascontiguousarray(nodes)
+        bricks = np.ascontiguousarray(bricks)
+        self._check(lib().arc_world_upload(self._h, g, _ptr(roots), _ptr(nodes), len(nodes), node_capacity,
+                                           _ptr(bricks), len(bricks), brick_capacity), "arc_world_upload")
+
+    def upload_world(self, world: HostWorld, **kw):
+        self.upload(*world.pools(), **kw)
+
+    def patch(self, patches) -> None:
+        arr = (GpuPatch * max(len(patches), 1))()
+        keep = []
+        for i, p in enumerate(patches):
+            nodes = np.ascontiguousarray(p["nodes"])
+            bricks = np.ascontiguousarray(p["bricks"])
+            keep += [nodes, bricks]
+            arr[i] = GpuPatch(p["chunk_flat_idx"], p["new_root_node_idx"], p["node_offset"], len(nodes),
+                              _ptr(nodes) if len(nodes) else None, p["brick_offset"], len(bricks),
+                              _ptr(bricks) if len(bricks) else None)
+        self._check(lib().arc_world_patch(self._h, arr, len(patches)), "arc_world_patch")
+
+    def set_launch_config(self, blocks_per_cu=0, block_threads=0, refill_lanes=0, batch_rays=0):
+        cfg = LaunchConfig(blocks_per_cu, block_threads, refill_lanes, batch_rays)
+        self._check(lib().arc_set_launch_config(self._h, C.byref(cfg)), "set_launch_config")
+
+    def launch_config(self) -> dict:
+        cfg = LaunchConfig()
+        self._check(lib().arc_get_launch_config(self._h, C.byref(cfg)), "get_launch_config")
+        return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}
+
+    def trace(self, params: TraceParams, rays: np.ndarray) -> np.ndarray:
+        """Host arrays in, compact hits out (synchronous)."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        if len(rays):
+            self._check(lib().arc_trace(self._h, C.byref(params), _ptr(rays), len(rays), _ptr(hits)), "arc_trace")
+        return hits
+
+    def trace_device(self, params: TraceParams, d_rays: int, n: int, d_hits: int, stream: int = 0) -> None:
+        """Device pointers (e.g. torch tensor data_ptr()), asynchronous on `stream`."""
+        self._check(lib().arc_trace_device(self._h, C.byref(params), d_rays, n, d_hits, stream or None),
+                    "arc_trace_device")
+
+    def trace_full_device(self, params: TraceParams, d_rays: int, n: int, d_out: int, stream: int = 0) -> None:
+        self._check(lib().arc_trace_full_device(self._h, C.byref(params), d_rays, n, d_out, stream or None),
+                    "arc_trace_full_device")

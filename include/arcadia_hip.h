@@ -1,0 +1,187 @@
+/*
+ * arcadia_hip.h — C ABI of the MI355X (gfx950) 64-tree DDA voxel ray traversal.
+ *
+ * The library keeps Arcadia's world memory layout byte for byte (Node64, LeafBrick64,
+ * flat chunk_roots grid) and replaces the WGSL ray cast `trace_world_64tree_from`
+ * (shaders/common/traversal64.wgsl:226-556) with hand-written HIP kernels. Every entry
+ * point returns ARC_OK (0) or a negative ARC_ERR_* code; no C++ exception crosses
+ * the ABI. arc_last_error() gives the text of the last failure on a context.
+ *
+ * Ownership: the caller owns host arrays; the context owns the device pools.
+ * A context is not thread-safe: use one per host thread or lock around it.
+ */
+#ifndef ARCADIA_HIP_H
+#define ARCADIA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------------- */
+#define ARC_OK 0
+#define ARC_ERR_INVALID (-1)   /* bad argument (null pointer, size out of range, bad enum) */
+#define ARC_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define ARC_ERR_NO_WORLD (-3)  /* trace before arc_world_upload */
+#define ARC_ERR_CAPACITY (-4)  /* patch or batch exceeds the device pool capacity */
+
+/* ---- layout constants (crates/arcadia-core/src/node64.rs:9-16, world_map.rs:16) ----- */
+#define ARC_IS_LEAF_FLAG 0x80000000u
+#define ARC_IS_UNIFORM_FLAG 0x40000000u
+#define ARC_PTR_INDEX_MASK 0x3FFFFFFFu
+#define ARC_EMPTY_CHUNK_SENTINEL 0xFFFFFFFFu
+#define ARC_CHUNK_DIM 256u
+#define ARC_MAT_AIR 0u
+#define ARC_MAT_WATER 18u
+
+/* Node64 (crates/arcadia-core/src/node64.rs:25-36, shaders/common/types.wgsl:65-70), 16 B. */
+typedef struct arc_node64 {
+  uint32_t mask_low;    /* child/voxel occupancy bits 0..31 */
+  uint32_t mask_high;   /* child/voxel occupancy bits 32..63 */
+  uint32_t child_ptr;   /* bit31 leaf, bit30 uniform, bits 0..29 node/brick base index */
+  uint32_t lod_payload; /* RGB8 in bits 0..23, material id in bits 24..31 */
+} arc_node64;
+
+/* LeafBrick64 (crates/arcadia-core/src/node64.rs:116-129), 64 B: one u8 material per voxel. */
+typedef struct arc_brick64 {
+  uint32_t packed_materials[16];
+} arc_brick64;
+
+/* One ray. ARC_SEM_WORLD takes meters (origin_m, dir, start_t_m, max_dist_m: the arguments
+ * of trace_world_64tree_from); ARC_SEM_CHUNK takes chunk-local voxel units (origin, dir,
+ * max_dist of raycast_chunk_tree, crates/arcadia-core/src/raycast_cpu.rs:28-33; t_start is
+ * ignored). 32 B. */
+typedef struct arc_ray {
+  float ox, oy, oz, t_start;
+  float dx, dy, dz, t_max;
+} arc_ray;
+
+/* Compact hit record, 16 B.
+ *  t_voxels: hit distance in voxel units (miss: max_dist*16 for ARC_SEM_WORLD, 0 for CHUNK)
+ *  vx,vy,vz: hit voxel in global voxel coordinates (chunk-local for ARC_SEM_CHUNK)
+ *  mat:      material id (0 on a miss)
+ *  face:     normal code: 2*axis + (component < 0), 6 + axis when the chosen axis has a zero
+ *            normal component (WGSL sign(0) == 0), ARC_FACE_NONE on a miss
+ *  steps:    outer DDA iterations that passed the loop guards (a first-step hit gives 1)
+ *  flags:    ARC_HIT_* bits */
+typedef struct arc_hit {
+  float t_voxels;
+  uint16_t vx, vy, vz;
+  uint8_t mat;
+  uint8_t face;
+  uint16_t steps;
+  uint16_t flags;
+} arc_hit;
+
+#define ARC_FACE_NONE 255u
+#define ARC_HIT_FLAG_HIT 1u
+#define ARC_HIT_FLAG_LOD 2u    /* hit resolved on a coarse cell (shift > 0) */
+
+/* Full per-ray result: the WGSL TraceResult (shaders/common/traversal64.wgsl:4-17) plus the
+ * step count and hit voxel, 80 B.
+ *  hit: bit 0 = hit; bits 8..15 = axis of the normal; bits 16..23 = shift of the hit cell
+ *       (0 = single voxel, 2/4/6 = collapsed or LOD cell)
+ *  hit_pos_m: meters for ARC_SEM_WORLD (voxel units for ARC_SEM_CHUNK) */
+typedef struct arc_trace_result {
+  float t_voxels;
+  float hit_pos_m[3];
+  float normal[3];
+  float lod_rgb[3];
+  float contact_ao;
+  uint32_t mat;
+  uint32_t voxel_hash;
+  uint32_t steps;
+  uint32_t hit;
+  uint32_t vx, vy, vz;
+  uint32_t reserved[2];
+} arc_trace_result;
+
+/* Traversal semantics. */
+#define ARC_SEM_WORLD 0u  /* trace_world_64tree_from (traversal64.wgsl:226-556), the GPU path */
+#define ARC_SEM_CHUNK 1u  /* raycast_chunk_tree (raycast_cpu.rs:28-218) on chunk 0 of the grid */
+
+/* Per-launch parameters: the traversal uniforms of CameraUniforms
+ * (shaders/common/types.wgsl:19-63; filled at crates/arcadia-render/src/renderer.rs:614-640). */
+typedef struct arc_trace_params {
+  uint32_t semantics;      /* ARC_SEM_* */
+  uint32_t lod_bias;       /* 0 primary, 1 secondary (water/mirror), >=2 GI */
+  uint32_t ignore_water;   /* skip water (mat 18) cells */
+  uint32_t max_steps[3];   /* ray_steps.xyz: cap for lod_bias 0, 1, >=2 */
+  float lod_dist_m[3];     /* lod_settings.xyz (meters) */
+  uint32_t lod_step[3];    /* quality_flags.x, quality_flags.y, (quality_flags.w >> 16) & 0xFF */
+  float ao_max_dist_bits;  /* point_light_count.w reinterpreted as f32 (contact AO range) */
+  uint32_t reserved[3];
+} arc_trace_params;
+
+/* A ray set inside one batched launch. Rays and hits are device pointers. */
+typedef struct arc_ray_set {
+  const arc_ray* rays;
+  arc_hit* hits;
+  uint64_t n;
+  arc_trace_params params;
+} arc_ray_set;
+
+/* Incremental pool update (crates/arcadia-world/src/world_map.rs:20-27). */
+typedef struct arc_gpu_patch {
+  uint32_t chunk_flat_idx;
+  uint32_t new_root_node_idx;
+  uint32_t node_offset;
+  uint32_t n_nodes;
+  const arc_node64* nodes;
+  uint32_t brick_offset;
+  uint32_t n_bricks;
+  const arc_brick64* bricks;
+} arc_gpu_patch;
+
+typedef struct arc_ctx arc_ctx;
+
+/* ---- context --------------------------------------------------------------------------- */
+/* Replaces the wgpu device/queue owned by Renderer (crates/arcadia-render/src/context.rs:25-146). */
+int arc_create(int device, arc_ctx** out);
+void arc_destroy(arc_ctx* ctx);
+const char* arc_last_error(const arc_ctx* ctx);
+/* Number of compute units of the context's device (sizes persistent launches). */
+int arc_device_info(const arc_ctx* ctx, uint32_t* n_cu, uint32_t* lds_bytes_per_block);
+
+/* ---- world pools ----------------------------------------------------------------------- */
+/* Upload chunk_roots / node_pool / brick_pool (the @group(0) bindings 1..3 created by
+ * create_group0_resources, crates/arcadia-render/src/gpu_pools.rs:124-261). Host pointers.
+ * node_capacity/brick_capacity >= n_nodes/n_bricks reserve headroom for later patches
+ * (0 = exact size). */
+int arc_world_upload(arc_ctx* ctx, const uint32_t grid[3], const uint32_t* chunk_roots,
+                     const arc_node64* nodes, uint64_t n_nodes, uint64_t node_capacity,
+                     const arc_brick64* bricks, uint64_t n_bricks, uint64_t brick_capacity);
+/* Apply patches in order (Renderer::apply_world_patches, crates/arcadia-render/src/renderer.rs:147-315):
+ * node/brick slices are written at their offsets, then chunk_roots[chunk_flat_idx] is set.
+ * Slices beyond the reserved capacity grow the pools (device copy), as the reference does. */
+int arc_world_patch(arc_ctx* ctx, const arc_gpu_patch* patches, uint64_t n);
+/* Device addresses of the pools (for callers that build their own launches). */
+int arc_world_device_pools(const arc_ctx* ctx, const uint32_t** roots, const arc_node64** nodes,
+                           const arc_brick64** bricks);
+
+/* ---- traversal ------------------------------------------------------------------------- */
+/* Host-pointer convenience: copies rays in, traces, copies hits out, synchronises. */
+int arc_trace(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* rays, uint64_t n,
+              arc_hit* hits_out);
+/* Device-pointer variant, asynchronous on `stream` (a hipStream_t; NULL = the null stream). */
+int arc_trace_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
+                     uint64_t n, arc_hit* d_hits, void* stream);
+/* Full TraceResult records (hit position, normal, lod rgb, voxel hash, contact AO). */
+int arc_trace_full_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
+                          uint64_t n, arc_trace_result* d_out, void* stream);
+/* Kernel launch shape of the persistent ray-queue kernel; 0 fields = autotuned defaults. */
+typedef struct arc_launch_config {
+  uint32_t blocks_per_cu;   /* resident workgroups per compute unit */
+  uint32_t block_threads;   /* 64..1024, multiple of 64 */
+  uint32_t refill_lanes;    /* a wave refills when at least this many lanes are idle (1..64) */
+  uint32_t batch_rays;      /* rays claimed per queue fetch (per wave) */
+} arc_launch_config;
+int arc_set_launch_config(arc_ctx* ctx, const arc_launch_config* cfg);
+int arc_get_launch_config(const arc_ctx* ctx, arc_launch_config* cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARCADIA_HIP_H */

@@ -1,0 +1,112 @@
+/*
+ * arcadia_world.h — host-side world building for the 64-tree traversal (C ABI).
+ *
+ * Produces the exact pools the traversal reads, with the reference's construction rules:
+ *   - per-chunk 4-level 64-tree: ChunkTree64::from_generator -> build_recursive ->
+ *     pack_build_tree (crates/arcadia-core/src/chunk_tree/builder.rs:86-378)
+ *   - global pools: WorldMap::rebuild_all_gpu_pools (crates/arcadia-world/src/world_map.rs:158-205)
+ *     with Morton chunk order, slot-0 dummy node/brick and pointer relocation (:689-735)
+ *   - the 256-entry material palette (crates/arcadia-core/src/material.rs:99-204)
+ * plus the seeded synthetic worlds and ray sets of the benchmark workloads.
+ * Return codes are the ARC_* codes of arcadia_hip.h.
+ */
+#ifndef ARCADIA_WORLD_H
+#define ARCADIA_WORLD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "arcadia_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct arc_world_host arc_world_host;
+
+/* A world of grid[0] x grid[1] x grid[2] chunks of 256^3 voxels, all empty. */
+int arc_world_host_create(const uint32_t grid[3], arc_world_host** out);
+void arc_world_host_destroy(arc_world_host* w);
+
+/* Replace chunk `chunk_flat_idx` (x + y*gx + z*gx*gy) by the tree built from a dense 256^3
+ * u8 material array (index (z*256 + y)*256 + x), with build_recursive's collapse rules. */
+int arc_world_host_set_chunk_dense(arc_world_host* w, uint32_t chunk_flat_idx, const uint8_t* dense);
+/* ChunkTree64::from_sparse_voxels (builder.rs:422-484): (lx,ly,lz,mat) quadruples, u32 each. */
+int arc_world_host_set_chunk_sparse(arc_world_host* w, uint32_t chunk_flat_idx,
+                                    const uint32_t* voxels_xyzm, uint64_t n);
+/* Local (unrelocated) tree of one chunk: node/brick counts and copies. */
+int arc_world_host_chunk_tree(const arc_world_host* w, uint32_t chunk_flat_idx, arc_node64* nodes,
+                              uint64_t* n_nodes, arc_brick64* bricks, uint64_t* n_bricks);
+
+/* Pack all chunks into the global pools (rebuild_all_gpu_pools). */
+int arc_world_host_pack(arc_world_host* w);
+/* Borrowed views of the packed pools (valid until the next pack/destroy). */
+int arc_world_host_pools(const arc_world_host* w, uint32_t grid[3], const uint32_t** roots,
+                         const arc_node64** nodes, uint64_t* n_nodes, const arc_brick64** bricks,
+                         uint64_t* n_bricks);
+/* The palette entry of `mat` as 8 floats (albedo rgb, roughness, emission rgb, variance). */
+int arc_palette_material(uint32_t mat, float out8[8]);
+/* Material::pack_lod_payload (material.rs:70-75). */
+uint32_t arc_palette_lod_payload(uint32_t mat);
+
+/* ---- seeded synthetic worlds --------------------------------------------------------- */
+#define ARC_GEN_HEIGHTMAP 0u /* 2D value-noise fBm heightmap, random mats per voxel        */
+#define ARC_GEN_CAVES 1u     /* 3D fBm terrain with caves + random axis-aligned cuboids      */
+#define ARC_GEN_SPARSE 2u    /* low heightmap terrain for large sparse worlds                */
+
+typedef struct arc_gen_params {
+  uint32_t kind;
+  uint32_t octaves;        /* fBm octaves */
+  uint64_t seed;
+  float height_min;        /* heightmap range, voxels */
+  float height_max;
+  float wavelength;        /* base noise wavelength, voxels */
+  float density_threshold; /* ARC_GEN_CAVES: solid when density > threshold */
+  uint32_t n_cuboids;      /* ARC_GEN_CAVES: random cuboids */
+  uint32_t cuboid_min;     /* cuboid edge range (voxels, inclusive) */
+  uint32_t cuboid_max;
+  uint32_t mat_min;        /* random material range (inclusive) */
+  uint32_t mat_max;
+} arc_gen_params;
+
+/* Fill `w` (all chunks) from `params`; chunks are built on `nthreads` threads. */
+int arc_world_generate(arc_world_host* w, const arc_gen_params* params, int nthreads);
+/* Material of global voxel (x,y,z) the generator defines (before tree building). */
+uint32_t arc_gen_voxel(const arc_world_host* w, const arc_gen_params* params, uint32_t x, uint32_t y,
+                       uint32_t z);
+
+/* ---- rays ------------------------------------------------------------------------------ */
+/* Pinhole camera in the reference convention (types.wgsl:334-345, primary_gbuffer.wgsl:102-115):
+ * forward/right/up unit vectors, tan of HALF the VERTICAL fov, aspect = width/height. */
+typedef struct arc_camera {
+  float eye[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  float tan_half_fov;
+  float aspect;
+  float jitter[2];
+} arc_camera;
+
+/* Camera looking along yaw/pitch (radians): forward = (cos p sin y, sin p, cos p cos y),
+ * right = normalize(cross(forward, +Y)), up = cross(right, forward). */
+int arc_camera_look(const float eye[3], float yaw, float pitch, float fov_y_deg, uint32_t width,
+                    uint32_t height, arc_camera* out);
+/* Primary rays of a width x height image in 8x8-tile order (a wave64 traces one tile).
+ * pixel_of_ray[i] = py*width + px (may be NULL). Ray fields: origin = eye (meters), dir,
+ * t_start = 0, t_max = max_dist_m. Direction: normalize(f + r*(ndc.x*aspect*tan) + u*(ndc.y*tan))
+ * with normalize(v) = v / sqrt(dot(v,v)). */
+int arc_gen_primary_rays(const arc_camera* cam, uint32_t width, uint32_t height, float max_dist_m,
+                         arc_ray* rays, uint32_t* pixel_of_ray);
+/* One cosine-weighted bounce ray per hit primary ray (lighting_restir_gi.wgsl:195-209):
+ * hit position = (origin*16 + dir*t_voxels) * 0.0625 m (the traversal's hit_pos_meters),
+ * normal from the hit's face code, origin hit + n*0.10 m, direction cosine-distributed around
+ * normalize(n + 0.22 y) with (u1,u2) from a seeded hash of (seed, ray index). Returns the
+ * number of rays written; src_index[k] = primary ray index of bounce ray k (may be NULL). */
+int64_t arc_gen_bounce_rays(const arc_ray* primary_rays, const arc_hit* primary_hits, uint64_t n,
+                            uint64_t seed, float max_dist_m, arc_ray* rays, uint32_t* src_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARCADIA_WORLD_H */

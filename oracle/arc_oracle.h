@@ -1,0 +1,83 @@
+/*
+ * CPU oracle for the 64-tree DDA traversal — TEST INFRASTRUCTURE ONLY.
+ *
+ * Plain-C restatement of the reference traversal, used as the checker of the HIP kernels
+ * (tests/, the smoke check and the CPU-baseline leg of bench.py). No product path links,
+ * loads or calls this library.
+ *
+ * Restated functions (each cites the reference lines it follows in arc_oracle.c):
+ *   orc_trace_world   trace_world_64tree_from   shaders/common/traversal64.wgsl:226-556
+ *   orc_trace_chunk   raycast_chunk_tree        crates/arcadia-core/src/raycast_cpu.rs:28-218
+ *   orc_trace_meters  WorldMap::raycast_meters  crates/arcadia-world/src/world_map.rs:464-545
+ *   orc_query_voxel   query_voxel_world_meters  shaders/common/traversal64.wgsl:182-220
+ * plus an independent checker that shares no code with the above:
+ *   orc_brute_dense   Amanatides-Woo unit-voxel walk in double precision over a dense grid.
+ *
+ * Pinning (what checks each function): the four reference KATs of raycast_cpu.rs:334-446;
+ * the dense brute-force walker (hit voxel/face away from ties, t within tolerance) on
+ * seeded worlds; closed forms for axis-aligned rays; agreement between orc_trace_world and
+ * orc_trace_chunk where the reference semantics coincide (one chunk, no LOD, no tiny
+ * direction components); and a second, pure-Python restatement (oracle/py_trace.py) that
+ * must match step counts bit for bit on small worlds. orc_trace_meters is pinned only by
+ * the KATs and the brute-force walker (its glam arithmetic is restated as per-lane IEEE
+ * ops; bit-level agreement with glam is unpinned).
+ */
+#ifndef ARC_ORACLE_H
+#define ARC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/arcadia_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_world {
+  uint32_t grid[3];
+  const uint32_t* roots;
+  const arc_node64* nodes;
+  uint64_t n_nodes;
+  const arc_brick64* bricks;
+  uint64_t n_bricks;
+} orc_world;
+
+/* Per-ray memory-traffic counters: loads the algorithm performs from each pool. */
+typedef struct orc_counters {
+  uint32_t node_loads;   /* node_pool[...] reads (root + descents; register-cached ancestors: 0) */
+  uint32_t brick_loads;  /* brick word reads */
+  uint32_t root_loads;   /* chunk_roots reads on a chunk-cache miss */
+  uint32_t steps;
+} orc_counters;
+
+/* All trace functions run over rays [0, n) on `nthreads` threads (<=0: one). */
+void orc_trace_world(const orc_world* w, const arc_trace_params* p, const arc_ray* rays,
+                     uint64_t n, arc_trace_result* out, orc_counters* counters, int nthreads);
+void orc_trace_chunk(const orc_world* w, const arc_ray* rays, uint64_t n, arc_trace_result* out,
+                     orc_counters* counters, int nthreads);
+void orc_trace_meters(const orc_world* w, const arc_ray* rays, uint64_t n, arc_trace_result* out,
+                      int nthreads);
+/* Compact records as the kernels write them. */
+void orc_compact(const arc_trace_result* full, uint64_t n, int world_units, arc_hit* out);
+
+uint32_t orc_query_voxel(const orc_world* w, uint32_t x, uint32_t y, uint32_t z);
+/* Dense material grid (x fastest) of the whole world, size (gx*256)*(gy*256)*(gz*256). */
+void orc_decode_dense(const orc_world* w, uint8_t* dense, int nthreads);
+/* Brute-force unit-voxel walk. Output per ray: hit (0/1), voxel, entry axis/sign code in
+ * face (2*axis + (dir>0)), t in voxel units (double), and tie = 1 when the entry plane is
+ * ambiguous to within 1e-9 relative (face then unchecked). */
+typedef struct orc_brute_hit {
+  double t;
+  int32_t vx, vy, vz;
+  int32_t hit;
+  int32_t face;
+  int32_t tie;
+  int32_t mat;
+  int32_t pad;
+} orc_brute_hit;
+void orc_brute_dense(const uint8_t* dense, const uint32_t dims[3], const arc_ray* rays, uint64_t n,
+                     int world_units, double max_t_scale, orc_brute_hit* out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

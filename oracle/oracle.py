@@ -1,0 +1,161 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, the smoke check and bench.py's CPU-baseline leg as the checker of the HIP
+traversal. Nothing in the product path imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+NODE_DTYPE = np.dtype([("mask_low", "<u4"), ("mask_high", "<u4"), ("child_ptr", "<u4"), ("lod_payload", "<u4")])
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("t_start", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("t_max", "<f4")])
+HIT_DTYPE = np.dtype([("t_voxels", "<f4"), ("vx", "<u2"), ("vy", "<u2"), ("vz", "<u2"), ("mat", "u1"),
+                      ("face", "u1"), ("steps", "<u2"), ("flags", "<u2")])
+RESULT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("normal", "<f4", (3,)),
+                         ("lod_rgb", "<f4", (3,)), ("contact_ao", "<f4"), ("mat", "<u4"), ("voxel_hash", "<u4"),
+                         ("steps", "<u4"), ("hit", "<u4"), ("vx", "<u4"), ("vy", "<u4"), ("vz", "<u4"),
+                         ("reserved", "<u4", (2,))])
+COUNTER_DTYPE = np.dtype([("node_loads", "<u4"), ("brick_loads", "<u4"), ("root_loads", "<u4"), ("steps", "<u4")])
+BRUTE_DTYPE = np.dtype([("t", "<f8"), ("vx", "<i4"), ("vy", "<i4"), ("vz", "<i4"), ("hit", "<i4"), ("face", "<i4"),
+                        ("tie", "<i4"), ("mat", "<i4"), ("pad", "<i4")])
+
+
+class OrcWorld(C.Structure):
+    _fields_ = [("grid", C.c_uint32 * 3), ("roots", C.c_void_p), ("nodes", C.c_void_p), ("n_nodes", C.c_uint64),
+                ("bricks", C.c_void_p), ("n_bricks", C.c_uint64)]
+
+
+class OrcParams(C.Structure):  # layout of arc_trace_params
+    _fields_ = [("semantics", C.c_uint32), ("lod_bias", C.c_uint32), ("ignore_water", C.c_uint32),
+                ("max_steps", C.c_uint32 * 3), ("lod_dist_m", C.c_float * 3), ("lod_step", C.c_uint32 * 3),
+                ("ao_max_dist_bits", C.c_float), ("reserved", C.c_uint32 * 3)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        h = C.CDLL(str(LIB_PATH))
+        P = C.c_void_p
+        h.orc_trace_world.argtypes = [P, P, P, C.c_uint64, P, P, C.c_int]
+        h.orc_trace_chunk.argtypes = [P, P, C.c_uint64, P, P, C.c_int]
+        h.orc_trace_meters.argtypes = [P, P, C.c_uint64, P, C.c_int]
+        h.orc_compact.argtypes = [P, C.c_uint64, C.c_int, P]
+        h.orc_query_voxel.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
+        h.orc_query_voxel.restype = C.c_uint32
+        h.orc_decode_dense.argtypes = [P, P, C.c_int]
+        h.orc_brute_dense.argtypes = [P, P, P, C.c_uint64, C.c_int, C.c_double, P, C.c_int]
+        for f in (h.orc_trace_world, h.orc_trace_chunk, h.orc_trace_meters, h.orc_compact, h.orc_decode_dense,
+                  h.orc_brute_dense):
+            f.restype = None
+        _lib = h
+    return _lib
+
+
+def _threads(n):
+    return n or min(16, os.cpu_count() or 1)
+
+
+class World:
+    """Oracle view of packed pools (keeps the arrays alive)."""
+
+    def __init__(self, grid, roots, nodes, bricks):
+        self.grid = tuple(int(g) for g in grid)
+        self.roots = np.ascontiguousarray(roots, np.uint32)
+        self.nodes = np.ascontiguousarray(nodes).view(NODE_DTYPE) if nodes.dtype != NODE_DTYPE else \
+            np.ascontiguousarray(nodes)
+        self.bricks = np.ascontiguousarray(bricks)
+        self._s = OrcWorld((C.c_uint32 * 3)(*self.grid), self.roots.ctypes.data, self.nodes.ctypes.data,
+                           len(self.nodes), self.bricks.ctypes.data, len(self.bricks))
+
+    @property
+    def ref(self):
+        return C.byref(self._s)
+
+
+def _params(p) -> OrcParams:
+    q = OrcParams()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(OrcParams))
+    return q
+
+
+def trace_world(world: World, params, rays: np.ndarray, nthreads: int = 0):
+    """trace_world_64tree_from over `rays` -> (full results, per-ray counters)."""
+    rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+    out = np.zeros(len(rays), RESULT_DTYPE)
+    cnt = np.zeros(len(rays), COUNTER_DTYPE)
+    q = _params(params)
+    if len(rays):
+        lib().orc_trace_world(world.ref, C.byref(q), rays.ctypes.data, len(rays), out.ctypes.data, cnt.ctypes.data,
+                              _threads(nthreads))
+    return out, cnt
+
+
+def trace_chunk(world: World, rays: np.ndarray, nthreads: int = 0):
+    """raycast_chunk_tree on chunk 0 -> (full results, counters)."""
+    rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+    out = np.zeros(len(rays), RESULT_DTYPE)
+    cnt = np.zeros(len(rays), COUNTER_DTYPE)
+    if len(rays):
+        lib().orc_trace_chunk(world.ref, rays.ctypes.data, len(rays), out.ctypes.data, cnt.ctypes.data,
+                              _threads(nthreads))
+    return out, cnt
+
+
+def trace_meters(world: World, rays: np.ndarray, nthreads: int = 0):
+    """WorldMap::raycast_meters -> full results (t in voxel units)."""
+    rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+    out = np.zeros(len(rays), RESULT_DTYPE)
+    if len(rays):
+        lib().orc_trace_meters(world.ref, rays.ctypes.data, len(rays), out.ctypes.data, _threads(nthreads))
+    return out
+
+
+def compact(full: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(full), HIT_DTYPE)
+    if len(full):
+        lib().orc_compact(np.ascontiguousarray(full).ctypes.data, len(full), 1, out.ctypes.data)
+    return out
+
+
+def query_voxel(world: World, x, y, z) -> int:
+    return int(lib().orc_query_voxel(world.ref, x, y, z))
+
+
+def decode_dense(world: World, nthreads: int = 0) -> np.ndarray:
+    X, Y, Z = (g * 256 for g in world.grid)
+    d = np.zeros((Z, Y, X), np.uint8)
+    lib().orc_decode_dense(world.ref, d.ctypes.data, _threads(nthreads))
+    return d
+
+
+def brute(dense: np.ndarray, rays: np.ndarray, world_units: bool = True, max_t_scale: float = 1.0,
+          nthreads: int = 0) -> np.ndarray:
+    """Independent dense unit-voxel walk (double precision)."""
+    dense = np.ascontiguousarray(dense, np.uint8)
+    Z, Y, X = dense.shape
+    dims = (C.c_uint32 * 3)(X, Y, Z)
+    rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+    out = np.zeros(len(rays), BRUTE_DTYPE)
+    if len(rays):
+        lib().orc_brute_dense(dense.ctypes.data, dims, rays.ctypes.data, len(rays), int(world_units), max_t_scale,
+                              out.ctypes.data, _threads(nthreads))
+    return out
