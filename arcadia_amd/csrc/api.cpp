@@ -53,12 +53,13 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail((ctx), ARC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+// blocks_per_cu 0 = as many workgroups as the kernel's occupancy keeps resident.
 arc_launch_config default_config() {
   arc_launch_config c;
-  c.blocks_per_cu = 8;
+  c.blocks_per_cu = 0;
   c.block_threads = 256;
-  c.refill_lanes = 16;
-  c.batch_rays = 64;
+  c.refill_lanes = 32;
+  c.batch_rays = 128;
   return c;
 }
 
@@ -172,7 +173,6 @@ int arc_set_launch_config(arc_ctx* c, const arc_launch_config* cfg) {
   if (!c || !cfg) return ARC_ERR_INVALID;
   arc_launch_config d = default_config();
   arc_launch_config n = *cfg;
-  if (n.blocks_per_cu == 0) n.blocks_per_cu = d.blocks_per_cu;
   if (n.block_threads == 0) n.block_threads = d.block_threads;
   if (n.refill_lanes == 0) n.refill_lanes = d.refill_lanes;
   if (n.batch_rays == 0) n.batch_rays = d.batch_rays;
@@ -317,7 +317,12 @@ static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_r
   a.n = n;
   a.full = full;
   a.queue = c->d_queue;
-  uint64_t want_blocks = (uint64_t)c->cfg.blocks_per_cu * c->n_cu;
+  uint32_t bpc = c->cfg.blocks_per_cu;
+  if (bpc == 0) {
+    hipError_t oe = arcdev::kernel_occupancy(params->semantics, full, c->cfg.block_threads, &bpc);
+    if (oe != hipSuccess) return fail(c, ARC_ERR_HIP, "occupancy query: %s", hipGetErrorString(oe));
+  }
+  uint64_t want_blocks = (uint64_t)bpc * c->n_cu;
   uint64_t max_useful = (n + c->cfg.block_threads - 1) / c->cfg.block_threads;
   a.blocks = (uint32_t)(want_blocks < max_useful ? want_blocks : max_useful);
   if (a.blocks == 0) a.blocks = 1;

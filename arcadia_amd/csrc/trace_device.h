@@ -106,20 +106,4 @@ __device__ inline float eval_leaf_ao(uint32_t ml, uint32_t mh, uint32_t cx, uint
   return clampf_(1.0f - occ, 0.35f, 1.0f);
 }
 
-// Per-lane traversal state.
-struct RayState {
-  float vo[3];    // origin, voxel units
-  float dir[3];   // direction used for inside points (WGSL: dir; CPU: safe_dir)
-  float rdir[3];  // raw direction (normal sign, AO)
-  float inv[3];
-  float noi[3];   // -vo * inv
-  float t, t_limit;
-  uint32_t vi[3], prev[3];
-  uint32_t mtf_k0, mtf_k1, mtf_r0, mtf_r1;  // 2-slot chunk cache (flat index keys)
-  uint32_t cached_shift;
-  uint32_t step;
-  uint4 l1, l2, l3;
-  uint32_t root_cpu;  // CPU semantics: chunk root index
-};
-
 }  // namespace arcdev

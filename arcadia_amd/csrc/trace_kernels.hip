@@ -1,12 +1,33 @@
 // 64-tree DDA traversal kernels for gfx950 and their launchers.
 //
-// Launch model: a persistent ray queue. The grid holds blocks_per_cu x n_cu workgroups;
-// every wave64 loops { refill idle lanes from a global atomic ray counter when at least
-// `refill` lanes are idle (one atomic per wave, lanes ranked with mbcnt over the ballot of
-// idle lanes); advance each live lane by one outer DDA iteration } until the queue is drained
-// and no lane is live. Results are written by ray index, so the schedule never changes an
-// output. The last workgroup to finish resets the queue counters, which keeps a launch
-// self-contained (no memset node, safe under HIP graph capture).
+// Execution model
+// ---------------
+// A persistent ray queue feeds a per-lane state machine. Each trip of a wave through its
+// loop advances every live lane by one *micro-step*:
+//   PH_CHECK  the lane holds the node of its current level: test the child bit, then either
+//             descend one level (issue one node load), resolve a leaf, or step the DDA over
+//             the empty cell / water node;
+//   PH_ROOT   consume a chunk_roots value, issue the chunk's root node load (or step over
+//             the empty chunk);
+//   PH_BRICK  consume a brick word, resolve the leaf material;
+//   PH_DONE   finished (hit or miss), waiting for the batched finalize + refill.
+// A DDA box step ends one outer iteration of trace_world_64tree_from; the next iteration's
+// restart node (cached ancestor, cached chunk root, or a chunk-cache lookup) is chosen in the
+// same micro-step. At most one dependent load is outstanding per lane per micro-step and all
+// lanes run the same short body, so the deep-descent lanes of a wave no longer hold back the
+// others, and the hit/miss record is written once per refill for all finished lanes together.
+//
+// Queue: rays are split into 8 contiguous segments, one per XCD (workgroups are dispatched
+// round-robin over the 8 XCDs, so blockIdx % 8 names the XCD); a wave claims windows of
+// `batch` rays from its home segment with one atomic and, once that segment is exhausted,
+// moves on to the next. Adjacent screen tiles therefore stay on one XCD's L2. Results are
+// written by ray index, so the schedule never changes an output. The last workgroup to finish
+// resets the counters, which keeps a launch self-contained (no memset node: graph-safe).
+//
+// Register budget: the hot lane state is scalar members plus one packed flags word, so it
+// lives entirely in VGPRs (-vo*inv is recomputed instead of held; negation is exact); the
+// cached ancestors, the chunk root node and the 2-slot chunk cache live in LDS, indexed by
+// level, so a restart reads one slot instead of selecting among registers.
 #include <hip/hip_runtime.h>
 
 #include "trace_device.h"
@@ -14,378 +35,447 @@
 
 namespace arcdev {
 
-template <uint32_t SEM>
-struct Sem;
+enum : uint32_t { PH_CHECK = 0u, PH_ROOT = 1u, PH_BRICK = 2u, PH_DONE = 3u };
+constexpr uint32_t kSegments = 8u;
 
-// ---------------------------------------------------------------------------------------
-// Results
-// ---------------------------------------------------------------------------------------
-struct Outcome {
-  bool hit;
-  float t_voxels;
-  uint32_t vi[3];
-  uint32_t mat;
-  uint32_t axis;
-  float nrm_axis;  // value of the normal on `axis` (-sign / -signum of dir)
-  uint32_t shift;
-  uint32_t steps;
-  // full-record extras
-  float hp[3];
-  uint32_t lod_payload;
-  float contact_ao;
+// packed flags word
+enum : uint32_t {
+  F_SHIFT = 0,   // 3 bits: level of `node` (6 root .. 0 leaf level)
+  F_CSHIFT = 3,  // 3 bits: cached_shift of the reference
+  F_MINSH = 6,   // 3 bits: LOD min_shift of the current iteration
+  F_PHASE = 9,   // 2 bits
+  F_HIT = 11,    // 1 bit
+  F_FRESH = 12,  // 1 bit: `node` was just loaded, record it in the ancestor cache
+  F_HASRAY = 13, // 1 bit: lane owns a ray whose record is not written yet
+  F_B = 16,      // 6 bits: child bit of a pending brick read
 };
 
-__device__ __forceinline__ void store_compact(arc_hit* out, uint32_t idx, const Outcome& o) {
-  uint4 v;
-  v.x = __float_as_uint(o.t_voxels);
-  uint32_t steps = o.steps > 0xFFFFu ? 0xFFFFu : o.steps;
-  if (o.hit) {
-    uint32_t face = o.nrm_axis > 0.0f ? 2u * o.axis : (o.nrm_axis < 0.0f ? 2u * o.axis + 1u : 6u + o.axis);
-    v.y = (o.vi[0] & 0xFFFFu) | (o.vi[1] << 16);
-    v.z = (o.vi[2] & 0xFFFFu) | ((o.mat & 0xFFu) << 16) | (face << 24);
-    v.w = steps | ((ARC_HIT_FLAG_HIT | (o.shift > 0 ? ARC_HIT_FLAG_LOD : 0u)) << 16);
+__device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
+__device__ __forceinline__ uint32_t fput(uint32_t st, uint32_t off, uint32_t w, uint32_t v) {
+  const uint32_t m = ((1u << w) - 1u) << off;
+  return (st & ~m) | ((v << off) & m);
+}
+
+struct Lane {
+  float ox, oy, oz;  // origin, voxel units
+  float dx, dy, dz;  // direction used for inside points (WORLD: dir, CHUNK: safe_dir)
+  float ix, iy, iz;  // 1 / safe_dir
+  float t, tl;       // t, t_limit
+  uint32_t vx, vy, vz;
+  uint32_t step;     // outer iterations begun
+  uint32_t st;       // packed flags
+  uint32_t pend;     // pending chunk_roots value / brick word; the material once a hit is found
+  uint4 node;        // node of the current level
+  uint32_t idx;
+};
+
+// Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
+//   slot 0: chunk root node, 1: cached level-1 ancestor, 2: cached level-2 ancestor
+//   (the slot of shift s is (6 - s) / 2; the level-3 ancestor is always `node` itself),
+//   slot 3: 2-slot chunk cache {k0, k1, r0, r1} (flat chunk index -> root node index).
+constexpr uint32_t kMaxBlock = 256u;
+constexpr uint32_t kMtfSlot = 3u;
+__device__ __forceinline__ uint32_t slot_of(uint32_t sh) { return (6u - sh) >> 1; }
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// t at which the ray crosses coordinate `c` on one axis: f32(c) * inv + (-o * inv).
+__device__ __forceinline__ float plane_t(uint32_t c, float o, float inv) {
+  const float q = (float)c * inv;
+  const float no = -o;
+  const float noi = no * inv;
+  return q + noi;
+}
+
+// LOD min_shift (traversal64.wgsl:355-370); `sidx` is the iteration index.
+template <uint32_t SEM>
+__device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, uint32_t sidx) {
+  if (SEM == ARC_SEM_CHUNK) return 0u;
+  uint32_t m = 0u;
+  if (p.lod_bias == 0u) {
+    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
+  } else if (p.lod_bias == 1u) {
+    if (t > p.lod_t_far || sidx > p.lod_step) m = 4u;
+    else if (t > p.lod_t) m = 2u;
   } else {
+    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
+  }
+  return m;
+}
+
+// Start of an outer iteration whose guards already passed: count it, then choose where the
+// descent restarts (traversal64.wgsl:311-387).
+template <uint32_t SEM>
+__device__ __forceinline__ void begin_iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C,
+                                                uint32_t diff) {
+  const uint32_t sidx = L.step;
+  L.step = sidx + 1u;
+  const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
+  uint32_t st = fput(L.st, F_MINSH, 3, minsh);
+  if (SEM == ARC_SEM_WORLD && diff >= 256u) {
+    const uint32_t key = (L.vx >> 8) + (L.vy >> 8) * w.gx + (L.vz >> 8) * w.gx * w.gy;
+    st = fput(st, F_CSHIFT, 3, 6u);
+    st = fput(st, F_PHASE, 2, PH_ROOT);
+    L.st = st;
+    const uint4 m = C[kMtfSlot * kMaxBlock];  // {k0, k1, r0, r1}
+    const bool slot1 = key == m.y;
+    // slot-1 hit: bubble to front; miss: evict slot 1, insert at slot 0 (root recorded on
+    // consume)
+    C[kMtfSlot * kMaxBlock] = make_uint4(key, m.x, slot1 ? m.w : ARC_EMPTY_CHUNK_SENTINEL, m.z);
+    L.pend = slot1 ? m.w : w.roots[key];
+    return;
+  }
+  const uint32_t cs = fget(st, F_CSHIFT, 3);
+  const bool c3 = diff < 4u && minsh == 0u && cs == 0u;
+  const bool c2 = !c3 && diff < 16u && minsh <= 2u && cs <= 2u;
+  const bool c1 = !c3 && !c2 && diff < 64u && minsh <= 4u && cs <= 4u;
+  const uint32_t sh = c3 ? 0u : (c2 ? 2u : (c1 ? 4u : 6u));
+  // `node` always equals the cached ancestor at cached_shift, so a restart at the level
+  // the last iteration ended on keeps it (this covers every level-3 restart, hence no
+  // level-3 cache register); otherwise the cached ancestor, or the chunk root held since its
+  // load (identical to re-reading it).
+  if (sh != cs) L.node = C[slot_of(sh) * kMaxBlock];
+  st = fput(st, F_SHIFT, 3, sh);
+  st = fput(st, F_CSHIFT, 3, sh);
+  st = fput(st, F_FRESH, 1, 0u);
+  L.st = fput(st, F_PHASE, 2, PH_CHECK);
+}
+
+// Guards at the top of an outer iteration (traversal64.wgsl:302-310 / raycast_cpu.rs:78-84).
+template <uint32_t SEM>
+__device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const DevParams& p, const Lane& L) {
+  const uint32_t max_steps = SEM == ARC_SEM_CHUNK ? 512u : p.max_steps;
+  const uint32_t wx = SEM == ARC_SEM_CHUNK ? 256u : (w.gx << 8);
+  const uint32_t wy = SEM == ARC_SEM_CHUNK ? 256u : (w.gy << 8);
+  const uint32_t wz = SEM == ARC_SEM_CHUNK ? 256u : (w.gz << 8);
+  return L.step >= max_steps || L.t >= L.tl || L.vx >= wx || L.vy >= wy || L.vz >= wz;
+}
+
+// One micro-step of a live lane.
+template <uint32_t SEM>
+__device__ __forceinline__ void micro_step(const DevWorld& w, const DevParams& p, Lane& L, uint4* C) {
+  bool do_box = false;
+  uint32_t span = 1u;
+  uint32_t mat = 0u;
+  bool resolve = false;
+  uint32_t st = L.st;
+  const uint32_t phase = fget(st, F_PHASE, 2);
+  if (phase == PH_CHECK) {
+    const uint4 node = L.node;
+    const uint32_t sh = fget(st, F_SHIFT, 3);
+    if (fget(st, F_FRESH, 1)) {  // record the node just loaded (traversal64.wgsl:544-551)
+      if (sh != 0u) C[slot_of(sh) * kMaxBlock] = node;
+      st = fput(st, F_FRESH, 1, 0u);
+    }
+    const uint32_t cx = (L.vx >> sh) & 3u, cy = (L.vy >> sh) & 3u, cz = (L.vz >> sh) & 3u;
+    const uint32_t b = (cz << 4) | (cy << 2) | cx;
+    if (SEM == ARC_SEM_WORLD && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
+        ((node.w >> 24) & 0xFFu) == ARC_MAT_WATER) {
+      do_box = true;
+      span = 4u << sh;
+    } else if (!has_child(node.x, node.y, b)) {
+      do_box = true;
+      span = 1u << sh;
+    } else {
+      // Level-3 nodes are leaves in every pool the upload check accepts; `sh == 0` only
+      // bounds the descent if a malformed node slipped in through a patch.
+      const bool is_leaf = (node.z & ARC_IS_LEAF_FLAG) != 0u;
+      if (is_leaf || sh == 0u || sh <= fget(st, F_MINSH, 3)) {
+        if (is_leaf && (node.z & ARC_IS_UNIFORM_FLAG) == 0u) {
+          L.pend = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
+          st = fput(st, F_B, 6, b);
+          st = fput(st, F_PHASE, 2, PH_BRICK);
+        } else {
+          mat = (node.w >> 24) & 0xFFu;
+          resolve = true;
+        }
+      } else {
+        L.node = w.nodes[child_offset(node, b)];
+        st = fput(st, F_SHIFT, 3, sh - 2u);
+        st = fput(st, F_CSHIFT, 3, sh - 2u);
+        st = fput(st, F_FRESH, 1, 1u);
+      }
+    }
+  } else if (phase == PH_ROOT) {
+    const uint32_t root = L.pend;
+    reinterpret_cast<uint32_t*>(C + kMtfSlot * kMaxBlock)[2] = root;  // r0
+    if (root == ARC_EMPTY_CHUNK_SENTINEL) {
+      do_box = true;
+      span = 256u;
+    } else {
+      L.node = w.nodes[root];
+      st = fput(st, F_SHIFT, 3, 6u);
+      st = fput(st, F_CSHIFT, 3, 6u);
+      st = fput(st, F_FRESH, 1, 1u);
+      st = fput(st, F_PHASE, 2, PH_CHECK);
+    }
+  } else {  // PH_BRICK
+    mat = (L.pend >> ((fget(st, F_B, 6) & 3u) * 8u)) & 0xFFu;
+    resolve = true;
+  }
+  if (resolve) {  // traversal64.wgsl:446-523 (the record itself is written at finalize)
+    span = 1u << fget(st, F_SHIFT, 3);
+    if (SEM == ARC_SEM_WORLD && p.ignore_water && mat == ARC_MAT_WATER) {
+      do_box = true;
+    } else if (mat != ARC_MAT_AIR) {
+      st = fput(st, F_PHASE, 2, PH_DONE);
+      st = fput(st, F_HIT, 1, 1u);
+      L.pend = mat;
+    } else {
+      do_box = true;
+    }
+  }
+  L.st = st;
+  if (do_box) {  // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292)
+    const uint32_t m = ~(span - 1u);
+    const uint32_t bx = L.vx & m, by = L.vy & m, bz = L.vz & m;
+    const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
+    const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
+    const float tx = plane_t(ex, L.ox, L.ix), ty = plane_t(ey, L.oy, L.iy), tz = plane_t(ez, L.oz, L.iz);
+    const float t_exit = fminf(fminf(tx, ty), tz);
+    const float next_t = fmaxf(L.t, t_exit);
+    const bool is_x = tx <= ty && tx <= tz;
+    const bool is_y = !is_x && (ty <= tz);
+    const bool is_z = !is_x && !is_y;
+    const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
+    const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
+    const uint32_t nx = is_x ? ex - (px ? 0u : 1u) : umin_(umax_(f2u(ipx), bx), bx + span - 1u);
+    const uint32_t ny = is_y ? ey - (py ? 0u : 1u) : umin_(umax_(f2u(ipy), by), by + span - 1u);
+    const uint32_t nz = is_z ? ez - (pz ? 0u : 1u) : umin_(umax_(f2u(ipz), bz), bz + span - 1u);
+    const uint32_t diff = (nx ^ L.vx) | (ny ^ L.vy) | (nz ^ L.vz);
+    L.vx = nx;
+    L.vy = ny;
+    L.vz = nz;
+    L.t = next_t;
+    if (iteration_guards_fail<SEM>(w, p, L)) {
+      L.st = fput(fput(L.st, F_PHASE, 2, PH_DONE), F_HIT, 1, 0u);
+    } else {
+      begin_iteration<SEM>(w, p, L, C, diff);
+    }
+  }
+}
+
+// Ray setup: traversal64.wgsl:234-300 (WORLD) / raycast_cpu.rs:34-76 (CHUNK), then the guards
+// and restart of iteration 0.
+template <uint32_t SEM>
+__device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, const arc_ray& r, Lane& L,
+                                         uint4* C) {
+  L.step = 0u;
+  L.st = fput(fput(0u, F_HASRAY, 1, 1u), F_PHASE, 2, PH_DONE);
+  if (SEM == ARC_SEM_WORLD) {
+    const float max_t_v = r.t_max * 16.0f;
+    const float start_t_v = fmaxf(r.t_start * 16.0f, 0.0f);
+    L.ox = r.ox * 16.0f;
+    L.oy = r.oy * 16.0f;
+    L.oz = r.oz * 16.0f;
+    L.dx = r.dx;
+    L.dy = r.dy;
+    L.dz = r.dz;
+    const float sx = fabsf(r.dx) < 1e-7f ? 1e-7f : r.dx;
+    const float sy = fabsf(r.dy) < 1e-7f ? 1e-7f : r.dy;
+    const float sz = fabsf(r.dz) < 1e-7f ? 1e-7f : r.dz;
+    L.ix = 1.0f / sx;
+    L.iy = 1.0f / sy;
+    L.iz = 1.0f / sz;
+    const float wx = (float)w.gx * 256.0f, wy = (float)w.gy * 256.0f, wz = (float)w.gz * 256.0f;
+    // intersect_aabb (traversal64.wgsl:134-142)
+    const float tnx = ((L.ix > 0.0f ? 0.0f : wx) - L.ox) * L.ix;
+    const float tny = ((L.iy > 0.0f ? 0.0f : wy) - L.oy) * L.iy;
+    const float tnz = ((L.iz > 0.0f ? 0.0f : wz) - L.oz) * L.iz;
+    const float txx = ((L.ix > 0.0f ? wx : 0.0f) - L.ox) * L.ix;
+    const float txy = ((L.iy > 0.0f ? wy : 0.0f) - L.oy) * L.iy;
+    const float txz = ((L.iz > 0.0f ? wz : 0.0f) - L.oz) * L.iz;
+    const float wh_x = fmaxf(fmaxf(tnx, tny), tnz);
+    const float wh_y = fminf(fminf(txx, txy), txz);
+    const float t_enter = fmaxf(wh_x, start_t_v);
+    if (wh_y < t_enter || t_enter >= max_t_v) return;
+    L.t = t_enter;
+    L.tl = fminf(wh_y, max_t_v);
+    const float tt = L.t + 1e-4f;
+    const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
+    L.vx = f2u(clampf_(L.ox + qx, 0.0f, wx - 1e-3f));
+    L.vy = f2u(clampf_(L.oy + qy, 0.0f, wy - 1e-3f));
+    L.vz = f2u(clampf_(L.oz + qz, 0.0f, wz - 1e-3f));
+    C[kMtfSlot * kMaxBlock] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, ARC_EMPTY_CHUNK_SENTINEL, ARC_EMPTY_CHUNK_SENTINEL);
+  } else {
+    const uint32_t root = w.roots[0];
+    if (root == ARC_EMPTY_CHUNK_SENTINEL) return;
+    const uint4 rn = w.nodes[root];
+    if (rn.x == 0u && rn.y == 0u) return;  // tree.is_empty()
+    L.ox = r.ox;
+    L.oy = r.oy;
+    L.oz = r.oz;
+    L.dx = fabsf(r.dx) > 1e-7f ? r.dx : copysignf(1e-7f, r.dx);
+    L.dy = fabsf(r.dy) > 1e-7f ? r.dy : copysignf(1e-7f, r.dy);
+    L.dz = fabsf(r.dz) > 1e-7f ? r.dz : copysignf(1e-7f, r.dz);
+    L.ix = 1.0f / L.dx;
+    L.iy = 1.0f / L.dy;
+    L.iz = 1.0f / L.dz;
+    // intersect_aabb (raycast_cpu.rs:222-234)
+    const float ax0 = (0.0f - L.ox) * L.ix, ax1 = (256.0f - L.ox) * L.ix;
+    const float ay0 = (0.0f - L.oy) * L.iy, ay1 = (256.0f - L.oy) * L.iy;
+    const float az0 = (0.0f - L.oz) * L.iz, az1 = (256.0f - L.oz) * L.iz;
+    const float tmin = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fminf(az0, az1));
+    const float tmax = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1));
+    if (!(tmax >= fmaxf(tmin, 0.0f))) return;
+    const float t_enter = fmaxf(tmin, 0.0f);
+    if (tmax < t_enter || t_enter >= r.t_max) return;
+    L.t = t_enter;
+    L.tl = fminf(tmax, r.t_max);
+    const float tt = L.t + 1e-4f;
+    const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
+    L.vx = f2u(clampf_(L.ox + qx, 0.0f, 256.0f - 1e-3f));
+    L.vy = f2u(clampf_(L.oy + qy, 0.0f, 256.0f - 1e-3f));
+    L.vz = f2u(clampf_(L.oz + qz, 0.0f, 256.0f - 1e-3f));
+    C[0] = rn;  // chunk root for every root restart
+  }
+  L.st = fput(L.st, F_CSHIFT, 3, 6u);
+  if (iteration_guards_fail<SEM>(w, p, L)) return;
+  begin_iteration<SEM>(w, p, L, C, 0xFFFFFFFFu);  // WORLD: chunk lookup; CHUNK: restart at the root
+}
+
+// ---------------------------------------------------------------------------------------
+// Finalize: hit resolution (traversal64.wgsl:467-523 / raycast_cpu.rs:162-185, 295-327)
+// ---------------------------------------------------------------------------------------
+struct HitInfo {
+  float exact_t;
+  uint32_t axis;
+  float nrm_axis;
+  float hx, hy, hz;  // hit point, voxel units
+};
+
+template <uint32_t SEM>
+__device__ __forceinline__ HitInfo resolve_hit(const Lane& L, float rdx, float rdy, float rdz) {
+  const uint32_t sh = fget(L.st, F_SHIFT, 3);
+  const uint32_t span = 1u << sh;
+  const uint32_t m = ~(span - 1u);
+  const uint32_t bx = L.vx & m, by = L.vy & m, bz = L.vz & m;
+  const float tx = plane_t(L.ix >= 0.0f ? bx : bx + span, L.ox, L.ix);
+  const float ty = plane_t(L.iy >= 0.0f ? by : by + span, L.oy, L.iy);
+  const float tz = plane_t(L.iz >= 0.0f ? bz : bz + span, L.oz, L.iz);
+  const float cell_enter = fmaxf(fmaxf(tx, ty), tz);
+  HitInfo h;
+  h.exact_t = fmaxf(cell_enter, 0.0f);
+  const float max_t = cell_enter;
+  const float eps = 1e-4f * fmaxf(max_t * 0.01f, 1.0f);
+  const float dtx = tx - max_t, dty = ty - max_t, dtz = tz - max_t;
+  const float cx = fabsf(dtx) <= eps ? fabsf(rdx) : 0.0f;
+  const float cy = fabsf(dty) <= eps ? fabsf(rdy) : 0.0f;
+  const float cz = fabsf(dtz) <= eps ? fabsf(rdz) : 0.0f;
+  uint32_t axis;
+  if (cx >= cy && cx >= cz && cx > 0.0f) axis = 0;
+  else if (cy >= cz && cy > 0.0f) axis = 1;
+  else if (cz > 0.0f) axis = 2;
+  else if (tx >= ty && tx >= tz) axis = 0;
+  else if (ty >= tz) axis = 1;
+  else axis = 2;
+  const float dax = axis == 0 ? rdx : (axis == 1 ? rdy : rdz);
+  h.axis = axis;
+  h.nrm_axis = SEM == ARC_SEM_WORLD ? -wgsl_sign(dax) : -rust_signum(dax);
+  const float qx = L.dx * h.exact_t, qy = L.dy * h.exact_t, qz = L.dz * h.exact_t;
+  h.hx = L.ox + qx;
+  h.hy = L.oy + qy;
+  h.hz = L.oz + qz;
+  return h;
+}
+
+template <uint32_t SEM>
+__device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays, const Lane& L) {
+  uint4 v;
+  const uint32_t steps = L.step > 0xFFFFu ? 0xFFFFu : L.step;
+  if (fget(L.st, F_HIT, 1)) {
+    float rdx = L.dx, rdy = L.dy, rdz = L.dz;
+    if (SEM == ARC_SEM_CHUNK) {  // raw direction: the normal and tie-break use it
+      const arc_ray& r = rays[L.idx];
+      rdx = r.dx;
+      rdy = r.dy;
+      rdz = r.dz;
+    }
+    const HitInfo h = resolve_hit<SEM>(L, rdx, rdy, rdz);
+    const uint32_t face = h.nrm_axis > 0.0f ? 2u * h.axis : (h.nrm_axis < 0.0f ? 2u * h.axis + 1u : 6u + h.axis);
+    v.x = __float_as_uint(h.exact_t);
+    v.y = (L.vx & 0xFFFFu) | (L.vy << 16);
+    v.z = (L.vz & 0xFFFFu) | ((L.pend & 0xFFu) << 16) | (face << 24);
+    v.w = steps | ((ARC_HIT_FLAG_HIT | (fget(L.st, F_SHIFT, 3) > 0u ? ARC_HIT_FLAG_LOD : 0u)) << 16);
+  } else {
+    float miss_t = 0.0f;
+    if (SEM == ARC_SEM_WORLD) miss_t = rays[L.idx].t_max * 16.0f;
+    v.x = __float_as_uint(miss_t);
     v.y = 0u;
     v.z = (uint32_t)ARC_FACE_NONE << 24;
     v.w = steps;
   }
-  reinterpret_cast<uint4*>(out)[idx] = v;
+  reinterpret_cast<uint4*>(out)[L.idx] = v;
 }
 
-// ---------------------------------------------------------------------------------------
-// World semantics: trace_world_64tree_from (traversal64.wgsl:226-556)
-// ---------------------------------------------------------------------------------------
-template <>
-struct Sem<ARC_SEM_WORLD> {
-  // Returns false when the ray is already finished (miss before the loop).
-  static __device__ __forceinline__ bool init(const DevWorld& w, const arc_ray& r, RayState& s, Outcome& o) {
-    const float o3[3] = {r.ox, r.oy, r.oz};
-    const float d3[3] = {r.dx, r.dy, r.dz};
-    const uint32_t g[3] = {w.gx, w.gy, w.gz};
-    o.hit = false;
-    o.t_voxels = r.t_max * 16.0f;
-    o.steps = 0;
-    o.contact_ao = 1.0f;
-    o.lod_payload = 0;
-    o.mat = 0;
-    o.shift = 0;
-    o.axis = 1;
-    o.nrm_axis = 1.0f;
-    for (int a = 0; a < 3; ++a) {
-      float m = d3[a] * r.t_max;
-      o.hp[a] = o3[a] + m;  // res.hit_pos_meters = origin_m + dir * max_dist_m
-      o.vi[a] = 0;
-    }
-    const float max_t_v = r.t_max * 16.0f;
-    const float start_t_v = fmaxf(r.t_start * 16.0f, 0.0f);
-    float tminv[3], tmaxv[3];
-    for (int a = 0; a < 3; ++a) {
-      s.vo[a] = o3[a] * 16.0f;
-      s.dir[a] = d3[a];
-      s.rdir[a] = d3[a];
-      float safe = fabsf(d3[a]) < 1e-7f ? 1e-7f : d3[a];
-      s.inv[a] = 1.0f / safe;
-      float wmax = (float)g[a] * 256.0f;
-      float eb = s.inv[a] > 0.0f ? 0.0f : wmax;
-      float xb = s.inv[a] > 0.0f ? wmax : 0.0f;
-      float e0 = eb - s.vo[a], x0 = xb - s.vo[a];
-      tminv[a] = e0 * s.inv[a];
-      tmaxv[a] = x0 * s.inv[a];
-    }
-    float wh_x = fmaxf(fmaxf(tminv[0], tminv[1]), tminv[2]);
-    float wh_y = fminf(fminf(tmaxv[0], tmaxv[1]), tmaxv[2]);
-    float t_enter = fmaxf(wh_x, start_t_v);
-    if (wh_y < t_enter || t_enter >= max_t_v) return false;
-    s.t = t_enter;
-    s.t_limit = fminf(wh_y, max_t_v);
-    for (int a = 0; a < 3; ++a) {
-      float nv = -s.vo[a];
-      s.noi[a] = nv * s.inv[a];
-      float wmax = (float)g[a] * 256.0f;
-      float tt = s.t + 1e-4f;
-      float dm = s.dir[a] * tt;
-      float sp = clampf_(s.vo[a] + dm, 0.0f, wmax - 1e-3f);
-      s.vi[a] = f2u(sp);
-      s.prev[a] = 0xFFFFFFFFu;
-    }
-    s.mtf_k0 = s.mtf_k1 = 0xFFFFFFFFu;
-    s.mtf_r0 = s.mtf_r1 = ARC_EMPTY_CHUNK_SENTINEL;
-    s.cached_shift = 6u;
-    s.step = 0;
-    s.l1 = s.l2 = s.l3 = make_uint4(0, 0, 0, 0);
-    return true;
-  }
-};
-
-// ---------------------------------------------------------------------------------------
-// Chunk semantics: raycast_chunk_tree (raycast_cpu.rs:28-218) on chunk 0
-// ---------------------------------------------------------------------------------------
-template <>
-struct Sem<ARC_SEM_CHUNK> {
-  static __device__ __forceinline__ bool init(const DevWorld& w, const arc_ray& r, RayState& s, Outcome& o) {
-    const float o3[3] = {r.ox, r.oy, r.oz};
-    const float d3[3] = {r.dx, r.dy, r.dz};
-    o.hit = false;
-    o.t_voxels = 0.0f;
-    o.steps = 0;
-    o.contact_ao = 1.0f;
-    o.lod_payload = 0;
-    o.mat = 0;
-    o.shift = 0;
-    o.axis = 0;
-    o.nrm_axis = 0.0f;
-    for (int a = 0; a < 3; ++a) {
-      o.hp[a] = 0.0f;
-      o.vi[a] = 0;
-    }
-    const uint32_t root = w.roots[0];
-    if (root == ARC_EMPTY_CHUNK_SENTINEL) return false;
-    const uint4 rn = w.nodes[root];
-    if (rn.x == 0u && rn.y == 0u) return false;  // tree.is_empty()
-    float tmn[3], tmx[3];
-    for (int a = 0; a < 3; ++a) {
-      s.vo[a] = o3[a];
-      s.rdir[a] = d3[a];
-      float safe = fabsf(d3[a]) > 1e-7f ? d3[a] : copysignf(1e-7f, d3[a]);
-      s.dir[a] = safe;
-      s.inv[a] = 1.0f / safe;
-      float a0 = 0.0f - o3[a], a1 = 256.0f - o3[a];
-      float t0 = a0 * s.inv[a], t1 = a1 * s.inv[a];
-      tmn[a] = fminf(t0, t1);
-      tmx[a] = fmaxf(t0, t1);
-    }
-    float tmin = fmaxf(fmaxf(tmn[0], tmn[1]), tmn[2]);
-    float tmax = fminf(fminf(tmx[0], tmx[1]), tmx[2]);
-    if (!(tmax >= fmaxf(tmin, 0.0f))) return false;
-    float t_enter = fmaxf(tmin, 0.0f);
-    if (tmax < t_enter || t_enter >= r.t_max) return false;
-    s.t = t_enter;
-    s.t_limit = fminf(tmax, r.t_max);
-    for (int a = 0; a < 3; ++a) {
-      float nv = -o3[a];
-      s.noi[a] = nv * s.inv[a];
-      float tt = s.t + 1e-4f;
-      float dm = s.dir[a] * tt;
-      float sp = clampf_(o3[a] + dm, 0.0f, 256.0f - 1e-3f);
-      s.vi[a] = f2u(sp);
-      s.prev[a] = 0xFFFFFFFFu;
-    }
-    s.root_cpu = root;
-    s.mtf_k0 = s.mtf_k1 = 0xFFFFFFFFu;
-    s.mtf_r0 = s.mtf_r1 = root;
-    s.cached_shift = 6u;
-    s.step = 0;
-    s.l1 = s.l2 = s.l3 = make_uint4(0, 0, 0, 0);
-    return true;
-  }
-};
-
-// One outer DDA iteration. Returns true when the ray finished (o filled).
-template <uint32_t SEM, bool FULL>
-__device__ __forceinline__ bool dda_iteration(const DevWorld& w, const DevParams& p, RayState& s, Outcome& o) {
-  const uint32_t max_steps = SEM == ARC_SEM_CHUNK ? 512u : p.max_steps;
-  const uint32_t wmx = SEM == ARC_SEM_CHUNK ? 256u : (w.gx << 8);
-  const uint32_t wmy = SEM == ARC_SEM_CHUNK ? 256u : (w.gy << 8);
-  const uint32_t wmz = SEM == ARC_SEM_CHUNK ? 256u : (w.gz << 8);
-  if (s.step >= max_steps || s.t >= s.t_limit || s.vi[0] >= wmx || s.vi[1] >= wmy || s.vi[2] >= wmz) {
-    o.steps = s.step;
-    return true;
-  }
-  const uint32_t step = s.step;
-  s.step = step + 1u;
-  const uint32_t diff = (s.prev[0] ^ s.vi[0]) | (s.prev[1] ^ s.vi[1]) | (s.prev[2] ^ s.vi[2]);
-  s.prev[0] = s.vi[0];
-  s.prev[1] = s.vi[1];
-  s.prev[2] = s.vi[2];
-
-  uint32_t span;  // edge of the box to step over when no hit
-  uint32_t root_idx;
-  uint32_t min_shift = 0u;
-  bool hit = false;
-  uint4 node = make_uint4(0, 0, 0, 0);
-  uint32_t shift = 6u, b = 0u, mat = 0u;
-  if (SEM == ARC_SEM_WORLD) {
-    root_idx = s.mtf_r0;
-    if (diff >= 256u) {  // traversal64.wgsl:316-335
-      const uint32_t key = (s.vi[0] >> 8) + (s.vi[1] >> 8) * w.gx + (s.vi[2] >> 8) * w.gx * w.gy;
-      s.cached_shift = 6u;
-      if (key == s.mtf_k1) {
-        root_idx = s.mtf_r1;
-      } else {
-        root_idx = w.roots[key];
+template <uint32_t SEM>
+__device__ __forceinline__ void store_full(arc_trace_result* out, const arc_ray* rays, const DevParams& p,
+                                           const Lane& L) {
+  const arc_ray r = rays[L.idx];
+  float t_vox, hp0, hp1, hp2, n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, rgb0 = 0.0f, rgb1 = 0.0f, rgb2 = 0.0f, ao = 1.0f;
+  uint32_t vh = 0u, hitw = 0u, mat = 0u;
+  const bool hit = fget(L.st, F_HIT, 1) != 0u;
+  if (hit) {
+    const HitInfo h = resolve_hit<SEM>(L, r.dx, r.dy, r.dz);
+    const uint32_t sh = fget(L.st, F_SHIFT, 3);
+    t_vox = h.exact_t;
+    n0 = h.axis == 0 ? h.nrm_axis : 0.0f;
+    n1 = h.axis == 1 ? h.nrm_axis : 0.0f;
+    n2 = h.axis == 2 ? h.nrm_axis : 0.0f;
+    mat = L.pend;
+    hitw = 1u | (h.axis << 8) | (sh << 16);
+    if (SEM == ARC_SEM_WORLD) {
+      hp0 = h.hx * 0.0625f;
+      hp1 = h.hy * 0.0625f;
+      hp2 = h.hz * 0.0625f;
+      const uint32_t lp = L.node.w;
+      rgb0 = (float)(lp & 0xFFu) / 255.0f;
+      rgb1 = (float)((lp >> 8) & 0xFFu) / 255.0f;
+      rgb2 = (float)((lp >> 16) & 0xFFu) / 255.0f;
+      vh = pcg_hash((L.vx * 73856093u) ^ (L.vy * 19349663u) ^ (L.vz * 83492791u));
+      if (sh == 0u && p.lod_bias == 0u) {  // contact AO, traversal64.wgsl:501-521
+        const float hit_dist_m = L.t * 0.0625f;
+        const float raw = p.ao_max_dist;
+        const float ao_max = raw < 1.0f ? 24.0f : raw;
+        if (hit_dist_m < ao_max) {
+          const float raw_ao =
+              eval_leaf_ao(L.node.x, L.node.y, L.vx & 3u, L.vy & 3u, L.vz & 3u, n0, n1, n2, h.hx, h.hy, h.hz);
+          const float ao_min = ao_max * 0.5f;
+          const float den = ao_max - ao_min;
+          const float num = hit_dist_m - ao_min;
+          const float tt = clampf_(num / den, 0.0f, 1.0f);
+          const float t2 = tt * tt;
+          const float tw = 2.0f * tt;
+          const float sm = t2 * (3.0f - tw);
+          const float wgt = 1.0f - sm;
+          const float omw = 1.0f - wgt;
+          const float a0 = 1.0f * omw;
+          const float a1 = raw_ao * wgt;
+          ao = a0 + a1;
+        }
       }
-      s.mtf_k1 = s.mtf_k0;
-      s.mtf_r1 = s.mtf_r0;
-      s.mtf_k0 = key;
-      s.mtf_r0 = root_idx;
-    }
-  } else {
-    root_idx = s.root_cpu;
-  }
-  if (SEM == ARC_SEM_WORLD && root_idx == ARC_EMPTY_CHUNK_SENTINEL) {
-    span = 256u;
-  } else {
-    if (SEM == ARC_SEM_WORLD) {  // :355-370
-      if (p.lod_bias == 0u) {
-        if (s.t > p.lod_t || step > p.lod_step) min_shift = 2u;
-      } else if (p.lod_bias == 1u) {
-        if (s.t > p.lod_t_far || step > p.lod_step) min_shift = 4u;
-        else if (s.t > p.lod_t) min_shift = 2u;
-      } else {
-        if (s.t > p.lod_t || step > p.lod_step) min_shift = 2u;
-      }
-    }
-    if (diff < 4u && min_shift == 0u && s.cached_shift == 0u) {
-      node = s.l3;
-      shift = 0u;
-    } else if (diff < 16u && min_shift <= 2u && s.cached_shift <= 2u) {
-      node = s.l2;
-      shift = 2u;
-    } else if (diff < 64u && min_shift <= 4u && s.cached_shift <= 4u) {
-      node = s.l1;
-      shift = 4u;
     } else {
-      node = w.nodes[root_idx];
-      shift = 6u;
+      hp0 = h.hx;
+      hp1 = h.hy;
+      hp2 = h.hz;
     }
-    s.cached_shift = shift;
-    const uint32_t lx = s.vi[0] & 255u, ly = s.vi[1] & 255u, lz = s.vi[2] & 255u;
-    for (;;) {  // :390-552, at most 4 levels
-      const uint32_t cx = (lx >> shift) & 3u, cy = (ly >> shift) & 3u, cz = (lz >> shift) & 3u;
-      b = (cz << 4) | (cy << 2) | cx;
-      if (SEM == ARC_SEM_WORLD && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
-          ((node.w >> 24) & 0xFFu) == ARC_MAT_WATER) {
-        span = 4u << shift;
-        break;
-      }
-      if (!has_child(node.x, node.y, b)) {
-        span = 1u << shift;
-        break;
-      }
-      // Level-3 nodes are leaves in every pool the builder or the upload check accepts; the
-      // shift test only bounds the descent if a patch ever slipped a malformed node in.
-      const bool is_leaf = (node.z & ARC_IS_LEAF_FLAG) != 0u || shift == 0u;
-      if (is_leaf || shift <= min_shift) {
-        mat = (node.w >> 24) & 0xFFu;
-        if (is_leaf && (node.z & ARC_IS_UNIFORM_FLAG) == 0u) {
-          const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
-          mat = (word >> ((b & 3u) * 8u)) & 0xFFu;
-        }
-        span = 1u << shift;
-        if (SEM == ARC_SEM_WORLD && p.ignore_water && mat == ARC_MAT_WATER) break;
-        if (mat != ARC_MAT_AIR) hit = true;
-        break;
-      }
-      const uint32_t next = child_offset(node, b);
-      node = w.nodes[next];
-      shift -= 2u;
-      s.cached_shift = shift;
-      if (shift == 4u) s.l1 = node;
-      else if (shift == 2u) s.l2 = node;
-      else if (shift == 0u) s.l3 = node;
-    }
+  } else if (SEM == ARC_SEM_WORLD) {
+    t_vox = r.t_max * 16.0f;
+    const float q0 = r.dx * r.t_max, q1 = r.dy * r.t_max, q2 = r.dz * r.t_max;
+    hp0 = r.ox + q0;
+    hp1 = r.oy + q1;
+    hp2 = r.oz + q2;
+    n1 = 1.0f;
+  } else {
+    t_vox = 0.0f;
+    hp0 = hp1 = hp2 = 0.0f;
   }
-
-  const uint32_t bmin0 = s.vi[0] & ~(span - 1u), bmin1 = s.vi[1] & ~(span - 1u), bmin2 = s.vi[2] & ~(span - 1u);
-  const uint32_t bmin[3] = {bmin0, bmin1, bmin2};
-  if (hit) {  // :467-523 / raycast_cpu.rs:162-185
-    float tm[3];
-    for (int a = 0; a < 3; ++a) {
-      const bool pos = s.inv[a] >= 0.0f;
-      const uint32_t ec = pos ? bmin[a] : bmin[a] + span;
-      float m = (float)ec * s.inv[a];
-      tm[a] = m + s.noi[a];
-    }
-    const float cell_enter = fmaxf(fmaxf(tm[0], tm[1]), tm[2]);
-    const float exact_t = fmaxf(cell_enter, 0.0f);
-    const float max_t = cell_enter;
-    const float eps = 1e-4f * fmaxf(max_t * 0.01f, 1.0f);
-    float cand[3];
-    for (int a = 0; a < 3; ++a) {
-      float dd = tm[a] - max_t;
-      cand[a] = fabsf(dd) <= eps ? fabsf(s.rdir[a]) : 0.0f;
-    }
-    uint32_t axis;
-    if (cand[0] >= cand[1] && cand[0] >= cand[2] && cand[0] > 0.0f) axis = 0;
-    else if (cand[1] >= cand[2] && cand[1] > 0.0f) axis = 1;
-    else if (cand[2] > 0.0f) axis = 2;
-    else if (tm[0] >= tm[1] && tm[0] >= tm[2]) axis = 0;
-    else if (tm[1] >= tm[2]) axis = 1;
-    else axis = 2;
-    const float dax = axis == 0 ? s.rdir[0] : (axis == 1 ? s.rdir[1] : s.rdir[2]);
-    o.hit = true;
-    o.t_voxels = exact_t;
-    o.vi[0] = s.vi[0];
-    o.vi[1] = s.vi[1];
-    o.vi[2] = s.vi[2];
-    o.mat = mat;
-    o.axis = axis;
-    o.nrm_axis = SEM == ARC_SEM_WORLD ? -wgsl_sign(dax) : -rust_signum(dax);
-    o.shift = shift;
-    o.steps = s.step;
-    if (FULL) {
-      float hpv[3];
-      for (int a = 0; a < 3; ++a) {
-        float m = s.dir[a] * exact_t;
-        hpv[a] = s.vo[a] + m;
-        o.hp[a] = SEM == ARC_SEM_WORLD ? hpv[a] * 0.0625f : hpv[a];
-      }
-      if (SEM == ARC_SEM_WORLD) {
-        o.lod_payload = node.w;
-        o.contact_ao = 1.0f;
-        if (shift == 0u && p.lod_bias == 0u) {  // :501-521
-          const float hit_dist_m = s.t * 0.0625f;
-          const float raw = p.ao_max_dist;
-          const float ao_max = raw < 1.0f ? 24.0f : raw;
-          if (hit_dist_m < ao_max) {
-            float n3[3] = {0.0f, 0.0f, 0.0f};
-            n3[axis] = o.nrm_axis;
-            const uint32_t cx = s.vi[0] & 3u, cy = s.vi[1] & 3u, cz = s.vi[2] & 3u;
-            const float raw_ao = eval_leaf_ao(node.x, node.y, cx, cy, cz, n3[0], n3[1], n3[2], hpv[0], hpv[1], hpv[2]);
-            const float ao_min = ao_max * 0.5f;
-            const float den = ao_max - ao_min;
-            const float num = hit_dist_m - ao_min;
-            const float tt = clampf_(num / den, 0.0f, 1.0f);
-            const float t2 = tt * tt;
-            const float tw = 2.0f * tt;
-            const float sm = t2 * (3.0f - tw);
-            const float wgt = 1.0f - sm;
-            const float omw = 1.0f - wgt;
-            const float a0 = 1.0f * omw;
-            const float a1 = raw_ao * wgt;
-            o.contact_ao = a0 + a1;
-          }
-        }
-      }
-    }
-    return true;
-  }
-
-  // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292)
-  float tmx[3];
-  uint32_t exitc[3];
-  for (int a = 0; a < 3; ++a) {
-    const bool pos = s.inv[a] >= 0.0f;
-    exitc[a] = pos ? bmin[a] + span : bmin[a];
-    float m = (float)exitc[a] * s.inv[a];
-    tmx[a] = m + s.noi[a];
-  }
-  const float t_exit = fminf(fminf(tmx[0], tmx[1]), tmx[2]);
-  const float next_t = fmaxf(s.t, t_exit);
-  const bool is_x = tmx[0] <= tmx[1] && tmx[0] <= tmx[2];
-  const bool is_y = !is_x && (tmx[1] <= tmx[2]);
-  const bool is_z = !is_x && !is_y;
-  const bool is_exit[3] = {is_x, is_y, is_z};
-  for (int a = 0; a < 3; ++a) {
-    const bool pos = s.inv[a] >= 0.0f;
-    const uint32_t outside = exitc[a] - (pos ? 0u : 1u);
-    float m = s.dir[a] * next_t;
-    const float ip = fmaxf(s.vo[a] + m, 0.0f);
-    const uint32_t inside = umin_(umax_(f2u(ip), bmin[a]), bmin[a] + span - 1u);
-    s.vi[a] = is_exit[a] ? outside : inside;
-  }
-  s.t = next_t;
-  return false;
+  uint4* q = reinterpret_cast<uint4*>(out + L.idx);
+  q[0] = make_uint4(__float_as_uint(t_vox), __float_as_uint(hp0), __float_as_uint(hp1), __float_as_uint(hp2));
+  q[1] = make_uint4(__float_as_uint(n0), __float_as_uint(n1), __float_as_uint(n2), __float_as_uint(rgb0));
+  q[2] = make_uint4(__float_as_uint(rgb1), __float_as_uint(rgb2), __float_as_uint(ao), mat);
+  q[3] = make_uint4(vh, L.step, hitw, hit ? L.vx : 0u);
+  q[4] = make_uint4(hit ? L.vy : 0u, hit ? L.vz : 0u, 0u, 0u);
 }
 
 template <bool FULL>
@@ -399,88 +489,88 @@ struct OutPtr<true> {
   using T = arc_trace_result;
 };
 
-template <uint32_t SEM>
-__device__ __forceinline__ void store_full(arc_trace_result* out, uint32_t idx, const Outcome& o) {
-  float nrm[3];
-  if (o.hit) {
-    nrm[0] = nrm[1] = nrm[2] = 0.0f;
-    nrm[o.axis] = o.nrm_axis;
-  } else if (SEM == ARC_SEM_WORLD) {
-    nrm[0] = 0.0f;
-    nrm[1] = 1.0f;
-    nrm[2] = 0.0f;
-  } else {
-    nrm[0] = nrm[1] = nrm[2] = 0.0f;
-  }
-  float rgb[3] = {0.0f, 0.0f, 0.0f};
-  uint32_t vh = 0u;
-  if (o.hit && SEM == ARC_SEM_WORLD) {
-    rgb[0] = (float)(o.lod_payload & 0xFFu) / 255.0f;
-    rgb[1] = (float)((o.lod_payload >> 8) & 0xFFu) / 255.0f;
-    rgb[2] = (float)((o.lod_payload >> 16) & 0xFFu) / 255.0f;
-    vh = pcg_hash((o.vi[0] * 73856093u) ^ (o.vi[1] * 19349663u) ^ (o.vi[2] * 83492791u));
-  }
-  uint4* q = reinterpret_cast<uint4*>(out + idx);
-  q[0] = make_uint4(__float_as_uint(o.t_voxels), __float_as_uint(o.hp[0]), __float_as_uint(o.hp[1]),
-                    __float_as_uint(o.hp[2]));
-  q[1] = make_uint4(__float_as_uint(nrm[0]), __float_as_uint(nrm[1]), __float_as_uint(nrm[2]), __float_as_uint(rgb[0]));
-  q[2] = make_uint4(__float_as_uint(rgb[1]), __float_as_uint(rgb[2]), __float_as_uint(o.contact_ao), o.mat);
-  const uint32_t hitw = o.hit ? (1u | (o.axis << 8) | (o.shift << 16)) : 0u;
-  q[3] = make_uint4(vh, o.steps, hitw, o.hit ? o.vi[0] : 0u);
-  q[4] = make_uint4(o.hit ? o.vi[1] : 0u, o.hit ? o.vi[2] : 0u, 0u, 0u);
+template <uint32_t SEM, bool FULL>
+__device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const arc_ray* rays, const DevParams& p,
+                                         const Lane& L) {
+  if (FULL) store_full<SEM>(reinterpret_cast<arc_trace_result*>(out), rays, p, L);
+  else store_compact<SEM>(reinterpret_cast<arc_hit*>(out), rays, L);
 }
 
 // Persistent ray-queue kernel.
+// queue[0..7]: rays claimed from each segment; queue[8]: finished workgroups.
 template <uint32_t SEM, bool FULL>
 __global__ __launch_bounds__(256) void k_trace(DevWorld w, DevParams p, const arc_ray* __restrict__ rays,
                                                typename OutPtr<FULL>::T* __restrict__ out, uint32_t n,
                                                uint32_t* __restrict__ queue, uint32_t refill, uint32_t batch) {
-  (void)batch;
   const uint32_t lane = __lane_id();
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-  RayState s;
-  Outcome o;
-  uint32_t idx = 0;
-  bool live = false;
+  __shared__ uint4 s_cold[4 * kMaxBlock];
+  uint4* const C = s_cold + threadIdx.x;
+  Lane L;
+  L.st = fput(0u, F_PHASE, 2, PH_DONE);
+  L.step = 0u;
+  L.idx = 0u;
+  // wave-uniform queue state
+  uint32_t seg = blockIdx.x % kSegments;
+  uint32_t exhausted = 0u;  // segments found exhausted so far
+  uint32_t wb = 0u, we = 0u;
   bool drained = false;
   for (;;) {
-    const uint64_t idle = __ballot(!live);
-    const uint32_t nidle = __popcll(idle);
-    if (!drained && (nidle >= refill || nidle == 64u)) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(queue, nidle);
-      base = __shfl(base, 0);
-      if (base + nidle >= n) drained = true;
-      if (!live) {
-        const uint32_t my = base + (uint32_t)__popcll(idle & lt_mask);
-        if (my < n) {
-          idx = my;
-          const arc_ray r = rays[my];
-          live = Sem<SEM>::init(w, r, s, o);
-          if (!live) {
-            if (FULL) store_full<SEM>(reinterpret_cast<arc_trace_result*>(out), my, o);
-            else store_compact(reinterpret_cast<arc_hit*>(out), my, o);
+    const bool idle_lane = fget(L.st, F_PHASE, 2) == PH_DONE;
+    const uint64_t idle = __ballot(idle_lane);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (drained) {
+      if (nidle == 64u) break;
+    } else if (nidle >= refill) {
+      // finalize finished lanes together, then refill every idle lane
+      if (idle_lane && fget(L.st, F_HASRAY, 1)) {
+        finalize<SEM, FULL>(out, rays, p, L);
+        L.st = fput(L.st, F_HASRAY, 1, 0u);
+      }
+      const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
+      uint32_t assigned = 0u;
+      uint32_t my = 0xFFFFFFFFu;
+      while (assigned < nidle) {
+        if (wb >= we) {
+          uint32_t got = 0u;
+          if (lane == 0) got = atomicAdd(queue + seg, batch);
+          got = rfl(__shfl(got, 0));
+          const uint32_t s_begin = (uint32_t)(((uint64_t)n * seg) / kSegments);
+          const uint32_t s_end = (uint32_t)(((uint64_t)n * (seg + 1u)) / kSegments);
+          const uint32_t lo = s_begin + umin_(got, s_end - s_begin);
+          if (lo >= s_end) {
+            seg = (seg + 1u) % kSegments;
+            if (++exhausted >= kSegments) {
+              drained = true;
+              break;
+            }
+            continue;
           }
+          wb = lo;
+          we = umin_(lo + batch, s_end);
         }
+        const uint32_t take = umin_(nidle - assigned, we - wb);
+        if (idle_lane && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);
+        wb += take;
+        assigned += take;
       }
-    }
-    if (drained && __ballot(live) == 0ull) break;
-    if (live) {
-      if (dda_iteration<SEM, FULL>(w, p, s, o)) {
-        live = false;
-        if (FULL) store_full<SEM>(reinterpret_cast<arc_trace_result*>(out), idx, o);
-        else store_compact(reinterpret_cast<arc_hit*>(out), idx, o);
+      if (my != 0xFFFFFFFFu) {
+        L.idx = my;
+        const arc_ray r = rays[my];
+        ray_init<SEM>(w, p, r, L, C);
       }
+      continue;  // re-ballot: freshly initialised lanes may already be done
     }
+    if (!idle_lane) micro_step<SEM>(w, p, L, C);
   }
+  if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L);
   // Last workgroup out resets the queue for the next launch on this stream.
   __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
-    const uint32_t done = atomicAdd(queue + 1, 1u);
+    const uint32_t done = atomicAdd(queue + kSegments, 1u);
     if (done == gridDim.x - 1u) {
-      atomicExch(queue, 0u);
-      atomicExch(queue + 1, 0u);
+      for (uint32_t s = 0; s <= kSegments; ++s) atomicExch(queue + s, 0u);
       __threadfence();
     }
   }
@@ -493,6 +583,34 @@ __global__ __launch_bounds__(256) void k_trace(DevWorld w, DevParams p, const ar
 // ---------------------------------------------------------------------------------------
 namespace arcdev {
 
+static DevParams make_params(const arc_trace_params& t) {
+  DevParams p;
+  p.lod_bias = t.lod_bias;
+  p.ignore_water = t.ignore_water;
+  const uint32_t cls = t.lod_bias == 0u ? 0u : (t.lod_bias == 1u ? 1u : 2u);
+  p.max_steps = t.max_steps[cls];
+  p.lod_step = t.lod_step[cls];
+  p.lod_t = t.lod_dist_m[cls] * 16.0f;
+  p.lod_t_far = (t.lod_dist_m[1] * 16.0f) * 2.2857f;
+  p.ao_max_dist = t.ao_max_dist_bits;
+  p.pad = 0;
+  return p;
+}
+
+template <uint32_t SEM, bool FULL>
+static const void* kernel_ptr() {
+  return reinterpret_cast<const void*>(&k_trace<SEM, FULL>);
+}
+
+hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_threads, uint32_t* blocks_per_cu) {
+  int nb = 0;
+  const void* f = semantics == ARC_SEM_CHUNK ? (full ? kernel_ptr<ARC_SEM_CHUNK, true>() : kernel_ptr<ARC_SEM_CHUNK, false>())
+                                             : (full ? kernel_ptr<ARC_SEM_WORLD, true>() : kernel_ptr<ARC_SEM_WORLD, false>());
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, (int)block_threads, 0);
+  *blocks_per_cu = nb > 0 ? (uint32_t)nb : 1u;
+  return e;
+}
+
 hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
   DevWorld w;
   w.gx = a.grid[0];
@@ -502,16 +620,7 @@ hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
   w.roots = a.roots;
   w.nodes = reinterpret_cast<const uint4*>(a.nodes);
   w.bricks = reinterpret_cast<const uint32_t*>(a.bricks);
-  DevParams p;
-  p.lod_bias = a.params.lod_bias;
-  p.ignore_water = a.params.ignore_water;
-  const uint32_t cls = a.params.lod_bias == 0u ? 0u : (a.params.lod_bias == 1u ? 1u : 2u);
-  p.max_steps = a.params.max_steps[cls];
-  p.lod_step = a.params.lod_step[cls];
-  p.lod_t = a.params.lod_dist_m[cls] * 16.0f;
-  p.lod_t_far = (a.params.lod_dist_m[1] * 16.0f) * 2.2857f;
-  p.ao_max_dist = a.params.ao_max_dist_bits;
-  p.pad = 0;
+  const DevParams p = make_params(a.params);
   const dim3 grid(a.blocks), block(a.block_threads);
   const uint32_t n = (uint32_t)a.n;
   if (a.params.semantics == ARC_SEM_CHUNK) {

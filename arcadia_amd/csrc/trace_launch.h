@@ -26,5 +26,7 @@ struct LaunchArgs {
 };
 
 hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream);
+// Resident workgroups per CU of the traversal kernel at `block_threads`.
+hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_threads, uint32_t* blocks_per_cu);
 
 }  // namespace arcdev

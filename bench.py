@@ -90,6 +90,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device copy of this many MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this traversal benchmark runs on exactly one GPU; --gpus must be 1 "
@@ -134,6 +136,12 @@ def main() -> int:
         if evs is not None:
             evs[2].record(stream)
 
+    if args.copy_probe_mb:
+        a = torch.ones(args.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

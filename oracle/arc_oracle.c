@@ -164,7 +164,8 @@ static void trace_world_one(const orc_world* w, const arc_trace_params* p, const
   const float o[3] = {ray->ox, ray->oy, ray->oz};
   const float dir[3] = {ray->dx, ray->dy, ray->dz};
   const float start_t_m = ray->t_start, max_dist_m = ray->t_max;
-  orc_counters c = {0, 0, 0, 0};
+  orc_counters c;
+  memset(&c, 0, sizeof c);
   result_init_world(res, o, dir, max_dist_m); /* :234-242 */
 
   float vo[3], safe[3], inv[3], wmax[3], noi[3];
@@ -261,6 +262,7 @@ static void trace_world_one(const orc_world* w, const arc_trace_params* p, const
       step_res st = step_dda_box(vo, dir, inv, noi, pos, bmin, bmax, t);
       t = st.t;
       memcpy(vi, st.vi, sizeof vi);
+      c.empty_chunk_steps++;
       continue;
     }
     uint32_t min_shift = 0u; /* :355-370 */
@@ -286,8 +288,10 @@ static void trace_world_one(const orc_world* w, const arc_trace_params* p, const
     } else {
       node = w->nodes[root_idx];
       c.node_loads++;
+      c.level_loads[0]++;
       shift = 6u;
     }
+    c.restarts[(6u - shift) / 2u]++;
     cached_shift = shift;
     const uint32_t l[3] = {vi[0] & 255u, vi[1] & 255u, vi[2] & 255u};
     for (;;) { /* :390-552 */
@@ -402,6 +406,7 @@ static void trace_world_one(const orc_world* w, const arc_trace_params* p, const
       node = w->nodes[next];
       c.node_loads++;
       shift -= 2u;
+      c.level_loads[(6u - shift) / 2u]++;
       cached_shift = shift;
       if (shift == 4u) l1 = node;
       else if (shift == 2u) l2 = node;
@@ -417,7 +422,8 @@ static void trace_world_one(const orc_world* w, const arc_trace_params* p, const
  * relocated image of the chunk's local nodes (world_map.rs:705-735). */
 static void trace_chunk_one(const orc_world* w, uint32_t root, const float o[3], const float d[3],
                             float max_dist, arc_trace_result* res, orc_counters* cnt) {
-  orc_counters c = {0, 0, 0, 0};
+  orc_counters c;
+  memset(&c, 0, sizeof c);
   memset(res, 0, sizeof(*res));
   res->contact_ao = 1.0f;
   if (root == ARC_EMPTY_CHUNK_SENTINEL ||

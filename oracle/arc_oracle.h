@@ -47,6 +47,9 @@ typedef struct orc_counters {
   uint32_t brick_loads;  /* brick word reads */
   uint32_t root_loads;   /* chunk_roots reads on a chunk-cache miss */
   uint32_t steps;
+  uint32_t level_loads[4]; /* node_loads split by tree level of the loaded node (0 = chunk root) */
+  uint32_t restarts[4];    /* outer iterations starting at level 0 (root) .. 3 (cached leaf) */
+  uint32_t empty_chunk_steps;
 } orc_counters;
 
 /* All trace functions run over rays [0, n) on `nthreads` threads (<=0: one). */

@@ -24,7 +24,8 @@ RESULT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("norm
                          ("lod_rgb", "<f4", (3,)), ("contact_ao", "<f4"), ("mat", "<u4"), ("voxel_hash", "<u4"),
                          ("steps", "<u4"), ("hit", "<u4"), ("vx", "<u4"), ("vy", "<u4"), ("vz", "<u4"),
                          ("reserved", "<u4", (2,))])
-COUNTER_DTYPE = np.dtype([("node_loads", "<u4"), ("brick_loads", "<u4"), ("root_loads", "<u4"), ("steps", "<u4")])
+COUNTER_DTYPE = np.dtype([("node_loads", "<u4"), ("brick_loads", "<u4"), ("root_loads", "<u4"), ("steps", "<u4"),
+                          ("level_loads", "<u4", (4,)), ("restarts", "<u4", (4,)), ("empty_chunk_steps", "<u4")])
 BRUTE_DTYPE = np.dtype([("t", "<f8"), ("vx", "<i4"), ("vy", "<i4"), ("vz", "<i4"), ("hit", "<i4"), ("face", "<i4"),
                         ("tie", "<i4"), ("mat", "<i4"), ("pad", "<i4")])
 
