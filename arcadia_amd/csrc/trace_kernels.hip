@@ -2,32 +2,30 @@
 //
 // Execution model
 // ---------------
-// A persistent ray queue feeds a per-lane state machine. Each trip of a wave through its
-// loop advances every live lane by one *micro-step*:
-//   PH_CHECK  the lane holds the node of its current level: test the child bit, then either
-//             descend one level (issue one node load), resolve a leaf, or step the DDA over
-//             the empty cell / water node;
-//   PH_ROOT   consume a chunk_roots value, issue the chunk's root node load (or step over
-//             the empty chunk);
-//   PH_BRICK  consume a brick word, resolve the leaf material;
-//   PH_DONE   finished (hit or miss), waiting for the batched finalize + refill.
-// A DDA box step ends one outer iteration of trace_world_64tree_from; the next iteration's
-// restart node (cached ancestor, cached chunk root, or a chunk-cache lookup) is chosen in the
-// same micro-step. At most one dependent load is outstanding per lane per micro-step and all
-// lanes run the same short body, so the deep-descent lanes of a wave no longer hold back the
-// others, and the hit/miss record is written once per refill for all finished lanes together.
+// A persistent ray queue feeds one ray per lane. Each trip of a wave through its loop runs one
+// outer iteration of trace_world_64tree_from (traversal64.wgsl:302-553) for every live lane:
+// choose the restart node, descend at most four levels, then either record a hit or take one
+// exact DDA box step. The traversal is VALU-issue bound (pointer chasing with a long
+// per-step float sequence), so the kernel is shaped to cut the instructions a wave issues:
+//   * finished lanes are not resolved in place: the hit record (cell entry planes, tie-break
+//     normal, hash, AO) is computed once per refill for all finished lanes together, not in
+//     every wave trip in which some lane of 64 hits;
+//   * cold per-lane state (cached ancestors, chunk root node, 2-slot chunk cache) lives in LDS
+//     indexed by level: a descent writes one slot, a restart reads one slot;
+//   * the lane keeps `diff` (XOR of the last step's voxels) instead of prev_vi, and keeps the
+//     node of its deepest level, which is the restart node whenever the restart level equals
+//     cached_shift (every level-3 restart).
+// Results are written by ray index, so neither the schedule nor these choices can change an
+// output: the cached-ancestor restart is equivalent to a root descent (the cached node at shift
+// s covers the current voxel's bits >= s+2 whenever the reference uses it).
 //
-// Queue: rays are split into 8 contiguous segments, one per XCD (workgroups are dispatched
-// round-robin over the 8 XCDs, so blockIdx % 8 names the XCD); a wave claims windows of
-// `batch` rays from its home segment with one atomic and, once that segment is exhausted,
-// moves on to the next. Adjacent screen tiles therefore stay on one XCD's L2. Results are
-// written by ray index, so the schedule never changes an output. The last workgroup to finish
-// resets the counters, which keeps a launch self-contained (no memset node: graph-safe).
-//
-// Register budget: the hot lane state is scalar members plus one packed flags word, so it
-// lives entirely in VGPRs (-vo*inv is recomputed instead of held; negation is exact); the
-// cached ancestors, the chunk root node and the 2-slot chunk cache live in LDS, indexed by
-// level, so a restart reads one slot instead of selecting among registers.
+// Queue: a wave claims windows of `batch` consecutive rays with one atomic and refills its
+// idle lanes from the window. The queue can be split into kSegments contiguous segments (each
+// workgroup starting on segment blockIdx % kSegments and moving on when it is exhausted); one
+// segment measured fastest on MI355X: with all waves sweeping the ray array together, the
+// concurrently traced rays form one coherent band of screen tiles instead of eight. The last
+// workgroup to finish resets the counters, which keeps a launch self-contained (no memset
+// node: graph-safe).
 #include <hip/hip_runtime.h>
 
 #include "trace_device.h"
@@ -35,19 +33,15 @@
 
 namespace arcdev {
 
-enum : uint32_t { PH_CHECK = 0u, PH_ROOT = 1u, PH_BRICK = 2u, PH_DONE = 3u };
-constexpr uint32_t kSegments = 8u;
+constexpr uint32_t kSegments = 1u;
 
-// packed flags word
+// packed per-lane flags
 enum : uint32_t {
-  F_SHIFT = 0,   // 3 bits: level of `node` (6 root .. 0 leaf level)
-  F_CSHIFT = 3,  // 3 bits: cached_shift of the reference
-  F_MINSH = 6,   // 3 bits: LOD min_shift of the current iteration
-  F_PHASE = 9,   // 2 bits
-  F_HIT = 11,    // 1 bit
-  F_FRESH = 12,  // 1 bit: `node` was just loaded, record it in the ancestor cache
-  F_HASRAY = 13, // 1 bit: lane owns a ray whose record is not written yet
-  F_B = 16,      // 6 bits: child bit of a pending brick read
+  F_CSHIFT = 0,  // 3 bits: cached_shift of the reference (level the last descent ended on)
+  F_SHIFT = 3,   // 3 bits: level of the hit cell (valid when done with a hit)
+  F_LIVE = 6,    // 1 bit: ray in flight
+  F_HIT = 7,     // 1 bit: finished with a hit
+  F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
 };
 
 __device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
@@ -60,18 +54,21 @@ struct Lane {
   float ox, oy, oz;  // origin, voxel units
   float dx, dy, dz;  // direction used for inside points (WORLD: dir, CHUNK: safe_dir)
   float ix, iy, iz;  // 1 / safe_dir
+  float nx, ny, nz;  // -origin * inv
   float t, tl;       // t, t_limit
   uint32_t vx, vy, vz;
+  uint32_t diff;     // prev_vi ^ vi of the coming iteration
   uint32_t step;     // outer iterations begun
   uint32_t st;       // packed flags
-  uint32_t pend;     // pending chunk_roots value / brick word; the material once a hit is found
-  uint4 node;        // node of the current level
+  uint32_t mat;      // material of the hit
+  uint4 node;        // node of the level the last descent ended on
   uint32_t idx;
 };
 
 // Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
-//   slot 0: chunk root node, 1: cached level-1 ancestor, 2: cached level-2 ancestor
-//   (the slot of shift s is (6 - s) / 2; the level-3 ancestor is always `node` itself),
+//   slot 0: chunk root node (after a hit: the node the hit resolved in),
+//   slot 1: cached level-1 ancestor, slot 2: cached level-2 ancestor (slot of shift s:
+//   (6 - s) / 2; the level-3 ancestor is the lane's `node`),
 //   slot 3: 2-slot chunk cache {k0, k1, r0, r1} (flat chunk index -> root node index).
 constexpr uint32_t kMaxBlock = 256u;
 constexpr uint32_t kMtfSlot = 3u;
@@ -79,11 +76,17 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t sh) { return (6u - sh) >> 1
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// float -> u32 with the saturating semantics of WGSL u32() / Rust `as` (negative and NaN -> 0,
+// too large -> 0xFFFFFFFF): exactly what v_cvt_u32_f32 does, in one instruction.
+__device__ __forceinline__ uint32_t cvt_u32(float x) {
+  uint32_t r;
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // t at which the ray crosses coordinate `c` on one axis: f32(c) * inv + (-o * inv).
-__device__ __forceinline__ float plane_t(uint32_t c, float o, float inv) {
+__device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
   const float q = (float)c * inv;
-  const float no = -o;
-  const float noi = no * inv;
   return q + noi;
 }
 
@@ -103,44 +106,6 @@ __device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, u
   return m;
 }
 
-// Start of an outer iteration whose guards already passed: count it, then choose where the
-// descent restarts (traversal64.wgsl:311-387).
-template <uint32_t SEM>
-__device__ __forceinline__ void begin_iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C,
-                                                uint32_t diff) {
-  const uint32_t sidx = L.step;
-  L.step = sidx + 1u;
-  const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
-  uint32_t st = fput(L.st, F_MINSH, 3, minsh);
-  if (SEM == ARC_SEM_WORLD && diff >= 256u) {
-    const uint32_t key = (L.vx >> 8) + (L.vy >> 8) * w.gx + (L.vz >> 8) * w.gx * w.gy;
-    st = fput(st, F_CSHIFT, 3, 6u);
-    st = fput(st, F_PHASE, 2, PH_ROOT);
-    L.st = st;
-    const uint4 m = C[kMtfSlot * kMaxBlock];  // {k0, k1, r0, r1}
-    const bool slot1 = key == m.y;
-    // slot-1 hit: bubble to front; miss: evict slot 1, insert at slot 0 (root recorded on
-    // consume)
-    C[kMtfSlot * kMaxBlock] = make_uint4(key, m.x, slot1 ? m.w : ARC_EMPTY_CHUNK_SENTINEL, m.z);
-    L.pend = slot1 ? m.w : w.roots[key];
-    return;
-  }
-  const uint32_t cs = fget(st, F_CSHIFT, 3);
-  const bool c3 = diff < 4u && minsh == 0u && cs == 0u;
-  const bool c2 = !c3 && diff < 16u && minsh <= 2u && cs <= 2u;
-  const bool c1 = !c3 && !c2 && diff < 64u && minsh <= 4u && cs <= 4u;
-  const uint32_t sh = c3 ? 0u : (c2 ? 2u : (c1 ? 4u : 6u));
-  // `node` always equals the cached ancestor at cached_shift, so a restart at the level
-  // the last iteration ended on keeps it (this covers every level-3 restart, hence no
-  // level-3 cache register); otherwise the cached ancestor, or the chunk root held since its
-  // load (identical to re-reading it).
-  if (sh != cs) L.node = C[slot_of(sh) * kMaxBlock];
-  st = fput(st, F_SHIFT, 3, sh);
-  st = fput(st, F_CSHIFT, 3, sh);
-  st = fput(st, F_FRESH, 1, 0u);
-  L.st = fput(st, F_PHASE, 2, PH_CHECK);
-}
-
 // Guards at the top of an outer iteration (traversal64.wgsl:302-310 / raycast_cpu.rs:78-84).
 template <uint32_t SEM>
 __device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const DevParams& p, const Lane& L) {
@@ -151,117 +116,112 @@ __device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const D
   return L.step >= max_steps || L.t >= L.tl || L.vx >= wx || L.vy >= wy || L.vz >= wz;
 }
 
-// One micro-step of a live lane.
+// One outer iteration of a live lane whose guards passed (traversal64.wgsl:311-552).
 template <uint32_t SEM>
-__device__ __forceinline__ void micro_step(const DevWorld& w, const DevParams& p, Lane& L, uint4* C) {
-  bool do_box = false;
-  uint32_t span = 1u;
-  uint32_t mat = 0u;
-  bool resolve = false;
+__device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C) {
+  const uint32_t sidx = L.step;
+  L.step = sidx + 1u;
+  const uint32_t diff = L.diff;
   uint32_t st = L.st;
-  const uint32_t phase = fget(st, F_PHASE, 2);
-  if (phase == PH_CHECK) {
-    const uint4 node = L.node;
-    const uint32_t sh = fget(st, F_SHIFT, 3);
-    if (fget(st, F_FRESH, 1)) {  // record the node just loaded (traversal64.wgsl:544-551)
-      if (sh != 0u) C[slot_of(sh) * kMaxBlock] = node;
-      st = fput(st, F_FRESH, 1, 0u);
-    }
-    const uint32_t cx = (L.vx >> sh) & 3u, cy = (L.vy >> sh) & 3u, cz = (L.vz >> sh) & 3u;
-    const uint32_t b = (cz << 4) | (cy << 2) | cx;
-    if (SEM == ARC_SEM_WORLD && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
-        ((node.w >> 24) & 0xFFu) == ARC_MAT_WATER) {
-      do_box = true;
-      span = 4u << sh;
-    } else if (!has_child(node.x, node.y, b)) {
-      do_box = true;
-      span = 1u << sh;
-    } else {
-      // Level-3 nodes are leaves in every pool the upload check accepts; `sh == 0` only
-      // bounds the descent if a malformed node slipped in through a patch.
-      const bool is_leaf = (node.z & ARC_IS_LEAF_FLAG) != 0u;
-      if (is_leaf || sh == 0u || sh <= fget(st, F_MINSH, 3)) {
-        if (is_leaf && (node.z & ARC_IS_UNIFORM_FLAG) == 0u) {
-          L.pend = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
-          st = fput(st, F_B, 6, b);
-          st = fput(st, F_PHASE, 2, PH_BRICK);
-        } else {
-          mat = (node.w >> 24) & 0xFFu;
-          resolve = true;
-        }
-      } else {
-        L.node = w.nodes[child_offset(node, b)];
-        st = fput(st, F_SHIFT, 3, sh - 2u);
-        st = fput(st, F_CSHIFT, 3, sh - 2u);
-        st = fput(st, F_FRESH, 1, 1u);
-      }
-    }
-  } else if (phase == PH_ROOT) {
-    const uint32_t root = L.pend;
-    reinterpret_cast<uint32_t*>(C + kMtfSlot * kMaxBlock)[2] = root;  // r0
+  uint32_t cs = fget(st, F_CSHIFT, 3);
+  uint32_t span = 256u;  // box to step over when the iteration ends without a hit
+  bool descend = true;
+  uint4 node = L.node;
+  uint32_t sh = 6u;
+  uint32_t minsh = 0u;
+  if (SEM == ARC_SEM_WORLD && diff >= 256u) {  // chunk change: 2-slot cache (:316-335)
+    const uint32_t key = (L.vx >> 8) + (L.vy >> 8) * w.gx + (L.vz >> 8) * w.gx * w.gy;
+    cs = 6u;
+    const uint4 m = C[kMtfSlot * kMaxBlock];  // {k0, k1, r0, r1}
+    const bool slot1 = key == m.y;
+    const uint32_t root = slot1 ? m.w : w.roots[key];
+    C[kMtfSlot * kMaxBlock] = make_uint4(key, m.x, root, m.z);
     if (root == ARC_EMPTY_CHUNK_SENTINEL) {
-      do_box = true;
-      span = 256u;
+      descend = false;  // step over the empty chunk (:337-352)
     } else {
-      L.node = w.nodes[root];
-      st = fput(st, F_SHIFT, 3, 6u);
-      st = fput(st, F_CSHIFT, 3, 6u);
-      st = fput(st, F_FRESH, 1, 1u);
-      st = fput(st, F_PHASE, 2, PH_CHECK);
-    }
-  } else {  // PH_BRICK
-    mat = (L.pend >> ((fget(st, F_B, 6) & 3u) * 8u)) & 0xFFu;
-    resolve = true;
-  }
-  if (resolve) {  // traversal64.wgsl:446-523 (the record itself is written at finalize)
-    span = 1u << fget(st, F_SHIFT, 3);
-    if (SEM == ARC_SEM_WORLD && p.ignore_water && mat == ARC_MAT_WATER) {
-      do_box = true;
-    } else if (mat != ARC_MAT_AIR) {
-      st = fput(st, F_PHASE, 2, PH_DONE);
-      st = fput(st, F_HIT, 1, 1u);
-      L.pend = mat;
-    } else {
-      do_box = true;
+      node = w.nodes[root];
+      C[0] = node;
     }
   }
+  if (descend) {
+    minsh = lod_min_shift<SEM>(p, L.t, sidx);
+    if (!(SEM == ARC_SEM_WORLD && diff >= 256u)) {  // ancestor restart (:374-386)
+      const bool c3 = diff < 4u && minsh == 0u && cs == 0u;
+      const bool c2 = !c3 && diff < 16u && minsh <= 2u && cs <= 2u;
+      const bool c1 = !c3 && !c2 && diff < 64u && minsh <= 4u && cs <= 4u;
+      sh = c3 ? 0u : (c2 ? 2u : (c1 ? 4u : 6u));
+      if (sh != cs) node = C[slot_of(sh) * kMaxBlock];
+      cs = sh;
+    }
+    const uint32_t lx = L.vx & 255u, ly = L.vy & 255u, lz = L.vz & 255u;
+    for (;;) {  // descent (:390-552), at most four levels
+      const uint32_t b = (((lz >> sh) & 3u) << 4) | (((ly >> sh) & 3u) << 2) | ((lx >> sh) & 3u);
+      if (SEM == ARC_SEM_WORLD && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
+          ((node.w >> 24) & 0xFFu) == ARC_MAT_WATER) {
+        span = 4u << sh;
+        break;
+      }
+      span = 1u << sh;
+      if (!has_child(node.x, node.y, b)) break;
+      // Level-3 nodes are leaves in every pool the upload check accepts; `sh == 0` only bounds
+      // the descent if a malformed node slipped in through a patch.
+      const bool is_leaf = (node.z & ARC_IS_LEAF_FLAG) != 0u;
+      if (is_leaf || sh == 0u || sh <= minsh) {
+        uint32_t mat = (node.w >> 24) & 0xFFu;
+        if (is_leaf && (node.z & ARC_IS_UNIFORM_FLAG) == 0u) {
+          const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
+          mat = (word >> ((b & 3u) * 8u)) & 0xFFu;
+        }
+        if (SEM == ARC_SEM_WORLD && p.ignore_water && mat == ARC_MAT_WATER) break;
+        if (mat != ARC_MAT_AIR) {  // hit: resolved at finalize (:467-523)
+          L.mat = mat;
+          L.node = node;
+          C[0] = node;
+          L.st = fput(fput(fput(fput(st, F_CSHIFT, 3, cs), F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
+          return;
+        }
+        break;
+      }
+      node = w.nodes[child_offset(node, b)];
+      sh -= 2u;
+      cs = sh;
+      if (sh != 0u) C[slot_of(sh) * kMaxBlock] = node;
+    }
+  }
+  L.node = node;
+  // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292)
+  const uint32_t mk = ~(span - 1u);
+  const uint32_t bx = L.vx & mk, by = L.vy & mk, bz = L.vz & mk;
+  const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
+  const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
+  const float tx = plane_t(ex, L.ix, L.nx), ty = plane_t(ey, L.iy, L.ny), tz = plane_t(ez, L.iz, L.nz);
+  const float t_exit = fminf(fminf(tx, ty), tz);
+  const float next_t = fmaxf(L.t, t_exit);
+  const bool is_x = tx <= ty && tx <= tz;
+  const bool is_y = !is_x && (ty <= tz);
+  const bool is_z = !is_x && !is_y;
+  const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
+  const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
+  const uint32_t nvx = is_x ? ex - (px ? 0u : 1u) : umin_(umax_(cvt_u32(ipx), bx), bx + span - 1u);
+  const uint32_t nvy = is_y ? ey - (py ? 0u : 1u) : umin_(umax_(cvt_u32(ipy), by), by + span - 1u);
+  const uint32_t nvz = is_z ? ez - (pz ? 0u : 1u) : umin_(umax_(cvt_u32(ipz), bz), bz + span - 1u);
+  L.diff = (nvx ^ L.vx) | (nvy ^ L.vy) | (nvz ^ L.vz);
+  L.vx = nvx;
+  L.vy = nvy;
+  L.vz = nvz;
+  L.t = next_t;
+  st = fput(st, F_CSHIFT, 3, cs);
+  if (iteration_guards_fail<SEM>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
   L.st = st;
-  if (do_box) {  // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292)
-    const uint32_t m = ~(span - 1u);
-    const uint32_t bx = L.vx & m, by = L.vy & m, bz = L.vz & m;
-    const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
-    const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
-    const float tx = plane_t(ex, L.ox, L.ix), ty = plane_t(ey, L.oy, L.iy), tz = plane_t(ez, L.oz, L.iz);
-    const float t_exit = fminf(fminf(tx, ty), tz);
-    const float next_t = fmaxf(L.t, t_exit);
-    const bool is_x = tx <= ty && tx <= tz;
-    const bool is_y = !is_x && (ty <= tz);
-    const bool is_z = !is_x && !is_y;
-    const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
-    const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
-    const uint32_t nx = is_x ? ex - (px ? 0u : 1u) : umin_(umax_(f2u(ipx), bx), bx + span - 1u);
-    const uint32_t ny = is_y ? ey - (py ? 0u : 1u) : umin_(umax_(f2u(ipy), by), by + span - 1u);
-    const uint32_t nz = is_z ? ez - (pz ? 0u : 1u) : umin_(umax_(f2u(ipz), bz), bz + span - 1u);
-    const uint32_t diff = (nx ^ L.vx) | (ny ^ L.vy) | (nz ^ L.vz);
-    L.vx = nx;
-    L.vy = ny;
-    L.vz = nz;
-    L.t = next_t;
-    if (iteration_guards_fail<SEM>(w, p, L)) {
-      L.st = fput(fput(L.st, F_PHASE, 2, PH_DONE), F_HIT, 1, 0u);
-    } else {
-      begin_iteration<SEM>(w, p, L, C, diff);
-    }
-  }
 }
 
 // Ray setup: traversal64.wgsl:234-300 (WORLD) / raycast_cpu.rs:34-76 (CHUNK), then the guards
-// and restart of iteration 0.
+// of iteration 0.
 template <uint32_t SEM>
 __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, const arc_ray& r, Lane& L,
                                          uint4* C) {
   L.step = 0u;
-  L.st = fput(fput(0u, F_HASRAY, 1, 1u), F_PHASE, 2, PH_DONE);
+  L.st = fput(0u, F_HASRAY, 1, 1u);  // not live until the setup succeeds
   if (SEM == ARC_SEM_WORLD) {
     const float max_t_v = r.t_max * 16.0f;
     const float start_t_v = fmaxf(r.t_start * 16.0f, 0.0f);
@@ -293,9 +253,9 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
     L.tl = fminf(wh_y, max_t_v);
     const float tt = L.t + 1e-4f;
     const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
-    L.vx = f2u(clampf_(L.ox + qx, 0.0f, wx - 1e-3f));
-    L.vy = f2u(clampf_(L.oy + qy, 0.0f, wy - 1e-3f));
-    L.vz = f2u(clampf_(L.oz + qz, 0.0f, wz - 1e-3f));
+    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, wx - 1e-3f));
+    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, wy - 1e-3f));
+    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, wz - 1e-3f));
     C[kMtfSlot * kMaxBlock] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, ARC_EMPTY_CHUNK_SENTINEL, ARC_EMPTY_CHUNK_SENTINEL);
   } else {
     const uint32_t root = w.roots[0];
@@ -324,14 +284,18 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
     L.tl = fminf(tmax, r.t_max);
     const float tt = L.t + 1e-4f;
     const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
-    L.vx = f2u(clampf_(L.ox + qx, 0.0f, 256.0f - 1e-3f));
-    L.vy = f2u(clampf_(L.oy + qy, 0.0f, 256.0f - 1e-3f));
-    L.vz = f2u(clampf_(L.oz + qz, 0.0f, 256.0f - 1e-3f));
-    C[0] = rn;  // chunk root for every root restart
+    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, 256.0f - 1e-3f));
+    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, 256.0f - 1e-3f));
+    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, 256.0f - 1e-3f));
+    C[0] = rn;    // chunk root for every root restart
+    L.node = rn;  // `node` == cached ancestor at cached_shift (6): iteration 0 restarts here
   }
-  L.st = fput(L.st, F_CSHIFT, 3, 6u);
+  L.nx = (-L.ox) * L.ix;
+  L.ny = (-L.oy) * L.iy;
+  L.nz = (-L.oz) * L.iz;
+  L.diff = 0xFFFFFFFFu;  // prev_vi starts at u32::MAX
   if (iteration_guards_fail<SEM>(w, p, L)) return;
-  begin_iteration<SEM>(w, p, L, C, 0xFFFFFFFFu);  // WORLD: chunk lookup; CHUNK: restart at the root
+  L.st = fput(fput(L.st, F_CSHIFT, 3, 6u), F_LIVE, 1, 1u);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -350,9 +314,9 @@ __device__ __forceinline__ HitInfo resolve_hit(const Lane& L, float rdx, float r
   const uint32_t span = 1u << sh;
   const uint32_t m = ~(span - 1u);
   const uint32_t bx = L.vx & m, by = L.vy & m, bz = L.vz & m;
-  const float tx = plane_t(L.ix >= 0.0f ? bx : bx + span, L.ox, L.ix);
-  const float ty = plane_t(L.iy >= 0.0f ? by : by + span, L.oy, L.iy);
-  const float tz = plane_t(L.iz >= 0.0f ? bz : bz + span, L.oz, L.iz);
+  const float tx = plane_t(L.ix >= 0.0f ? bx : bx + span, L.ix, L.nx);
+  const float ty = plane_t(L.iy >= 0.0f ? by : by + span, L.iy, L.ny);
+  const float tz = plane_t(L.iz >= 0.0f ? bz : bz + span, L.iz, L.nz);
   const float cell_enter = fmaxf(fmaxf(tx, ty), tz);
   HitInfo h;
   h.exact_t = fmaxf(cell_enter, 0.0f);
@@ -395,7 +359,7 @@ __device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays,
     const uint32_t face = h.nrm_axis > 0.0f ? 2u * h.axis : (h.nrm_axis < 0.0f ? 2u * h.axis + 1u : 6u + h.axis);
     v.x = __float_as_uint(h.exact_t);
     v.y = (L.vx & 0xFFFFu) | (L.vy << 16);
-    v.z = (L.vz & 0xFFFFu) | ((L.pend & 0xFFu) << 16) | (face << 24);
+    v.z = (L.vz & 0xFFFFu) | ((L.mat & 0xFFu) << 16) | (face << 24);
     v.w = steps | ((ARC_HIT_FLAG_HIT | (fget(L.st, F_SHIFT, 3) > 0u ? ARC_HIT_FLAG_LOD : 0u)) << 16);
   } else {
     float miss_t = 0.0f;
@@ -410,7 +374,7 @@ __device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays,
 
 template <uint32_t SEM>
 __device__ __forceinline__ void store_full(arc_trace_result* out, const arc_ray* rays, const DevParams& p,
-                                           const Lane& L) {
+                                           const Lane& L, const uint4* C) {
   const arc_ray r = rays[L.idx];
   float t_vox, hp0, hp1, hp2, n0 = 0.0f, n1 = 0.0f, n2 = 0.0f, rgb0 = 0.0f, rgb1 = 0.0f, rgb2 = 0.0f, ao = 1.0f;
   uint32_t vh = 0u, hitw = 0u, mat = 0u;
@@ -418,17 +382,18 @@ __device__ __forceinline__ void store_full(arc_trace_result* out, const arc_ray*
   if (hit) {
     const HitInfo h = resolve_hit<SEM>(L, r.dx, r.dy, r.dz);
     const uint32_t sh = fget(L.st, F_SHIFT, 3);
+    const uint4 node = C[0];  // the node the hit resolved in
     t_vox = h.exact_t;
     n0 = h.axis == 0 ? h.nrm_axis : 0.0f;
     n1 = h.axis == 1 ? h.nrm_axis : 0.0f;
     n2 = h.axis == 2 ? h.nrm_axis : 0.0f;
-    mat = L.pend;
+    mat = L.mat;
     hitw = 1u | (h.axis << 8) | (sh << 16);
     if (SEM == ARC_SEM_WORLD) {
       hp0 = h.hx * 0.0625f;
       hp1 = h.hy * 0.0625f;
       hp2 = h.hz * 0.0625f;
-      const uint32_t lp = L.node.w;
+      const uint32_t lp = node.w;
       rgb0 = (float)(lp & 0xFFu) / 255.0f;
       rgb1 = (float)((lp >> 8) & 0xFFu) / 255.0f;
       rgb2 = (float)((lp >> 16) & 0xFFu) / 255.0f;
@@ -439,7 +404,7 @@ __device__ __forceinline__ void store_full(arc_trace_result* out, const arc_ray*
         const float ao_max = raw < 1.0f ? 24.0f : raw;
         if (hit_dist_m < ao_max) {
           const float raw_ao =
-              eval_leaf_ao(L.node.x, L.node.y, L.vx & 3u, L.vy & 3u, L.vz & 3u, n0, n1, n2, h.hx, h.hy, h.hz);
+              eval_leaf_ao(node.x, node.y, L.vx & 3u, L.vy & 3u, L.vz & 3u, n0, n1, n2, h.hx, h.hy, h.hz);
           const float ao_min = ao_max * 0.5f;
           const float den = ao_max - ao_min;
           const float num = hit_dist_m - ao_min;
@@ -491,23 +456,26 @@ struct OutPtr<true> {
 
 template <uint32_t SEM, bool FULL>
 __device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const arc_ray* rays, const DevParams& p,
-                                         const Lane& L) {
-  if (FULL) store_full<SEM>(reinterpret_cast<arc_trace_result*>(out), rays, p, L);
+                                         const Lane& L, const uint4* C) {
+  if (FULL) store_full<SEM>(reinterpret_cast<arc_trace_result*>(out), rays, p, L, C);
   else store_compact<SEM>(reinterpret_cast<arc_hit*>(out), rays, L);
 }
 
 // Persistent ray-queue kernel.
-// queue[0..7]: rays claimed from each segment; queue[8]: finished workgroups.
+// queue[0..kSegments-1]: rays claimed from each segment; queue[kSegments]: finished workgroups.
+// Occupancy target: the compact variants fit 64 VGPRs (8 waves/SIMD) without spilling; the
+// full-record variants carry more live values through finalize and keep the default budget.
 template <uint32_t SEM, bool FULL>
-__global__ __launch_bounds__(256) void k_trace(DevWorld w, DevParams p, const arc_ray* __restrict__ rays,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 8, 8))) void k_trace(
+    DevWorld w, DevParams p, const arc_ray* __restrict__ rays,
                                                typename OutPtr<FULL>::T* __restrict__ out, uint32_t n,
                                                uint32_t* __restrict__ queue, uint32_t refill, uint32_t batch) {
-  const uint32_t lane = __lane_id();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   __shared__ uint4 s_cold[4 * kMaxBlock];
   uint4* const C = s_cold + threadIdx.x;
+  const uint32_t lane = __lane_id();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   Lane L;
-  L.st = fput(0u, F_PHASE, 2, PH_DONE);
+  L.st = 0u;
   L.step = 0u;
   L.idx = 0u;
   // wave-uniform queue state
@@ -516,7 +484,7 @@ __global__ __launch_bounds__(256) void k_trace(DevWorld w, DevParams p, const ar
   uint32_t wb = 0u, we = 0u;
   bool drained = false;
   for (;;) {
-    const bool idle_lane = fget(L.st, F_PHASE, 2) == PH_DONE;
+    const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (drained) {
@@ -524,7 +492,7 @@ __global__ __launch_bounds__(256) void k_trace(DevWorld w, DevParams p, const ar
     } else if (nidle >= refill) {
       // finalize finished lanes together, then refill every idle lane
       if (idle_lane && fget(L.st, F_HASRAY, 1)) {
-        finalize<SEM, FULL>(out, rays, p, L);
+        finalize<SEM, FULL>(out, rays, p, L, C);
         L.st = fput(L.st, F_HASRAY, 1, 0u);
       }
       const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
@@ -561,9 +529,9 @@ __global__ __launch_bounds__(256) void k_trace(DevWorld w, DevParams p, const ar
       }
       continue;  // re-ballot: freshly initialised lanes may already be done
     }
-    if (!idle_lane) micro_step<SEM>(w, p, L, C);
+    if (!idle_lane) iteration<SEM>(w, p, L, C);
   }
-  if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L);
+  if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C);
   // Last workgroup out resets the queue for the next launch on this stream.
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -27,6 +27,19 @@ METRIC = "Grays/s 64-tree DDA (primary+diffuse); cache rel-MSE vs FP64 oracle at
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic():
+    """HBM bytes per launch of the traversal kernel on this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary in profiles/ (scaled by the copy-kernel factors recorded
+    there); None when the summary is absent."""
+    p = ROOT / "profiles" / "k_trace_fullframe_traffic.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def build_workload(seed: int, nthreads: int):
     import arcadia_amd as A
 
@@ -201,7 +214,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "bytes_per_ray": {"primary": round(bytes_per["primary"], 2), "bounce": round(bytes_per["bounce"], 2)},
             "avg_launch_ms": round(launch_ms, 4),
         }
