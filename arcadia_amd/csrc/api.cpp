@@ -2,6 +2,7 @@
 // Host code only (built with hipcc as HIP so it can call the launchers).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -30,7 +31,7 @@ struct arc_ctx {
   arc_ray* d_rays = nullptr;
   arc_hit* d_hits = nullptr;
   uint64_t scratch_n = 0;
-  arc_launch_config cfg{};
+  arc_launch_config cfg[3]{};  // per ray class (lod_bias 0, 1, >=2)
 };
 
 namespace {
@@ -58,10 +59,12 @@ arc_launch_config default_config() {
   arc_launch_config c;
   c.blocks_per_cu = 0;
   c.block_threads = 256;
-  c.refill_lanes = 32;
-  c.batch_rays = 128;
+  c.refill_lanes = 24;
+  c.batch_rays = 64;
   return c;
 }
+
+uint32_t lod_class(const arc_trace_params* p) { return p->lod_bias == 0u ? 0u : (p->lod_bias == 1u ? 1u : 2u); }
 
 // Structural check of uploaded pools: every reachable node's children lie inside the node
 // pool, mixed leaves point inside the brick pool, and trees are at most four levels deep with
@@ -122,7 +125,7 @@ int arc_create(int device, arc_ctx** out) {
   arc_ctx* c = new (std::nothrow) arc_ctx();
   if (!c) return ARC_ERR_INVALID;
   c->device = device;
-  c->cfg = default_config();
+  for (auto& k : c->cfg) k = default_config();
   if (hipSetDevice(device) != hipSuccess) {
     delete c;
     return ARC_ERR_HIP;
@@ -169,8 +172,7 @@ int arc_device_info(const arc_ctx* c, uint32_t* n_cu, uint32_t* lds) {
   return ARC_OK;
 }
 
-int arc_set_launch_config(arc_ctx* c, const arc_launch_config* cfg) {
-  if (!c || !cfg) return ARC_ERR_INVALID;
+static int normalize_config(arc_ctx* c, const arc_launch_config* cfg, arc_launch_config* out) {
   arc_launch_config d = default_config();
   arc_launch_config n = *cfg;
   if (n.block_threads == 0) n.block_threads = d.block_threads;
@@ -178,14 +180,35 @@ int arc_set_launch_config(arc_ctx* c, const arc_launch_config* cfg) {
   if (n.batch_rays == 0) n.batch_rays = d.batch_rays;
   if (n.block_threads % 64 != 0 || n.block_threads > 256) return fail(c, ARC_ERR_INVALID, "block_threads must be 64..256, a multiple of 64");
   if (n.refill_lanes > 64) return fail(c, ARC_ERR_INVALID, "refill_lanes must be 1..64");
-  if (n.blocks_per_cu > 64) return fail(c, ARC_ERR_INVALID, "blocks_per_cu must be 1..64");
-  c->cfg = n;
+  if (n.blocks_per_cu > 64) return fail(c, ARC_ERR_INVALID, "blocks_per_cu must be 0..64");
+  if (n.batch_rays > (1u << 20)) return fail(c, ARC_ERR_INVALID, "batch_rays must be 1..2^20");
+  *out = n;
+  return ARC_OK;
+}
+
+int arc_set_launch_config(arc_ctx* c, const arc_launch_config* cfg) {
+  if (!c || !cfg) return ARC_ERR_INVALID;
+  arc_launch_config n;
+  int rc = normalize_config(c, cfg, &n);
+  if (rc != ARC_OK) return rc;
+  for (auto& k : c->cfg) k = n;
   return ARC_OK;
 }
 
 int arc_get_launch_config(const arc_ctx* c, arc_launch_config* cfg) {
   if (!c || !cfg) return ARC_ERR_INVALID;
-  *cfg = c->cfg;
+  *cfg = c->cfg[0];
+  return ARC_OK;
+}
+
+int arc_set_launch_config_class(arc_ctx* c, uint32_t cls, const arc_launch_config* cfg) {
+  if (!c || !cfg || cls > 2) return ARC_ERR_INVALID;
+  return normalize_config(c, cfg, &c->cfg[cls]);
+}
+
+int arc_get_launch_config_class(const arc_ctx* c, uint32_t cls, arc_launch_config* cfg) {
+  if (!c || !cfg || cls > 2) return ARC_ERR_INVALID;
+  *cfg = c->cfg[cls];
   return ARC_OK;
 }
 
@@ -317,18 +340,19 @@ static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_r
   a.n = n;
   a.full = full;
   a.queue = c->d_queue;
-  uint32_t bpc = c->cfg.blocks_per_cu;
+  const arc_launch_config& cfg = c->cfg[lod_class(params)];
+  uint32_t bpc = cfg.blocks_per_cu;
   if (bpc == 0) {
-    hipError_t oe = arcdev::kernel_occupancy(params->semantics, full, c->cfg.block_threads, &bpc);
+    hipError_t oe = arcdev::kernel_occupancy(params->semantics, full, cfg.block_threads, &bpc);
     if (oe != hipSuccess) return fail(c, ARC_ERR_HIP, "occupancy query: %s", hipGetErrorString(oe));
   }
   uint64_t want_blocks = (uint64_t)bpc * c->n_cu;
-  uint64_t max_useful = (n + c->cfg.block_threads - 1) / c->cfg.block_threads;
+  uint64_t max_useful = (n + cfg.block_threads - 1) / cfg.block_threads;
   a.blocks = (uint32_t)(want_blocks < max_useful ? want_blocks : max_useful);
   if (a.blocks == 0) a.blocks = 1;
-  a.block_threads = c->cfg.block_threads;
-  a.refill = c->cfg.refill_lanes;
-  a.batch = c->cfg.batch_rays;
+  a.block_threads = cfg.block_threads;
+  a.refill = cfg.refill_lanes;
+  a.batch = cfg.batch_rays;
   hipError_t e = arcdev::launch_trace(a, (hipStream_t)stream);
   if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
   return ARC_OK;
@@ -342,6 +366,65 @@ int arc_trace_device(arc_ctx* c, const arc_trace_params* params, const arc_ray* 
 int arc_trace_full_device(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n,
                           arc_trace_result* d_out, void* stream) {
   return launch(c, params, d_rays, n, d_out, true, stream);
+}
+
+int arc_autotune(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n, arc_hit* d_hits,
+                 void* stream, uint32_t reps, arc_launch_config* best, float* best_ms) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!valid_params(params)) return fail(c, ARC_ERR_INVALID, "invalid trace params");
+  if (n == 0 || !d_rays || !d_hits) return fail(c, ARC_ERR_INVALID, "autotune needs a non-empty ray set");
+  if (reps == 0) reps = 3;
+  if (reps > 64) reps = 64;
+  HIP_TRY(c, hipSetDevice(c->device));
+  const uint32_t cls = lod_class(params);
+  const arc_launch_config saved = c->cfg[cls];
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  // Candidate grid: workgroup size, resident workgroups per CU (0 = occupancy limit), refill
+  // threshold (idle lanes before a wave refills) and ray-window size per queue claim.
+  static const uint32_t kThreads[] = {128, 256};
+  static const uint32_t kBlocks[] = {0, 4, 6};
+  static const uint32_t kRefill[] = {8, 16, 24, 32, 48};
+  static const uint32_t kBatch[] = {32, 64, 128};
+  arc_launch_config win = saved;
+  float win_ms = 1e30f;
+  int rc = ARC_OK;
+  std::vector<float> t(reps);
+  for (uint32_t bt : kThreads)
+    for (uint32_t bpc : kBlocks)
+      for (uint32_t rf : kRefill)
+        for (uint32_t bat : kBatch) {
+          if (rc != ARC_OK) break;
+          c->cfg[cls] = arc_launch_config{bpc, bt, rf, bat};
+          rc = launch(c, params, d_rays, n, d_hits, false, stream);  // warm-up
+          for (uint32_t r = 0; r < reps && rc == ARC_OK; ++r) {
+            if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK) rc = launch(c, params, d_rays, n, d_hits, false, stream);
+            if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
+          }
+          if (rc != ARC_OK) break;
+          std::vector<float> sorted = t;
+          std::sort(sorted.begin(), sorted.end());
+          const float med = sorted[reps / 2];
+          if (med < win_ms) {
+            win_ms = med;
+            win = c->cfg[cls];
+          }
+        }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != ARC_OK) {
+    c->cfg[cls] = saved;
+    return rc;
+  }
+  c->cfg[cls] = win;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return ARC_OK;
 }
 
 int arc_trace(arc_ctx* c, const arc_trace_params* params, const arc_ray* rays, uint64_t n, arc_hit* hits_out) {

@@ -76,13 +76,11 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t sh) { return (6u - sh) >> 1
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// float -> u32 with the saturating semantics of WGSL u32() / Rust `as` (negative and NaN -> 0,
-// too large -> 0xFFFFFFFF): exactly what v_cvt_u32_f32 does, in one instruction.
-__device__ __forceinline__ uint32_t cvt_u32(float x) {
-  uint32_t r;
-  asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
-  return r;
-}
+// float -> u32 for a value already clamped to x >= 0 (never NaN), whose result is then clamped
+// into a box below 2^32: min(x, 4294967040) (the largest float below 2^32) keeps the conversion
+// defined and gives the same clamped result as the saturating WGSL u32() / Rust `as`. Plain
+// arithmetic (v_min_f32 + v_cvt_u32_f32), so the compiler may speculate it into selects.
+__device__ __forceinline__ uint32_t cvt_u32(float x) { return (uint32_t)fminf(x, 4294967040.0f); }
 
 // t at which the ray crosses coordinate `c` on one axis: f32(c) * inv + (-o * inv).
 __device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {

@@ -128,6 +128,10 @@ _SIGS = [
                                         C.c_void_p]),
     ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
+    ("arc_get_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
+    ("arc_autotune", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                               C.c_uint32, C.POINTER(LaunchConfig), C.POINTER(C.c_float)]),
     ("arc_world_host_create", C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     ("arc_world_host_destroy", None, [C.c_void_p]),
     ("arc_world_host_set_chunk_dense", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
@@ -388,10 +392,22 @@ class Context:
         cfg = LaunchConfig(blocks_per_cu, block_threads, refill_lanes, batch_rays)
         self._check(lib().arc_set_launch_config(self._h, C.byref(cfg)), "set_launch_config")
 
-    def launch_config(self) -> dict:
+    def launch_config(self, lod_class: int = 0) -> dict:
         cfg = LaunchConfig()
-        self._check(lib().arc_get_launch_config(self._h, C.byref(cfg)), "get_launch_config")
+        self._check(lib().arc_get_launch_config_class(self._h, lod_class, C.byref(cfg)), "get_launch_config")
         return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}
+
+    def set_launch_config_class(self, lod_class: int, blocks_per_cu=0, block_threads=0, refill_lanes=0,
+                                batch_rays=0):
+        cfg = LaunchConfig(blocks_per_cu, block_threads, refill_lanes, batch_rays)
+        self._check(lib().arc_set_launch_config_class(self._h, lod_class, C.byref(cfg)), "set_launch_config_class")
+
+    def autotune(self, params: TraceParams, d_rays: int, n: int, d_hits: int, stream: int = 0, reps: int = 3):
+        """Native autotuner: picks the fastest launch shape for the ray class of `params`."""
+        cfg, ms = LaunchConfig(), C.c_float()
+        self._check(lib().arc_autotune(self._h, C.byref(params), d_rays, n, d_hits, stream or None, reps,
+                                       C.byref(cfg), C.byref(ms)), "arc_autotune")
+        return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}, ms.value
 
     def trace(self, params: TraceParams, rays: np.ndarray) -> np.ndarray:
         """Host arrays in, compact hits out (synchronous)."""

@@ -103,6 +103,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default launch shapes")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -111,7 +112,7 @@ def main() -> int:
                                    "(rays shard by screen tile across GPUs with a full world replica each)"}))
         return 2
     if args.profile:
-        args.steps, args.warmup, args.no_cpu_baseline = min(args.steps, 5), 1, True
+        args.steps, args.warmup, args.no_cpu_baseline, args.no_autotune = min(args.steps, 5), 1, True, True
 
     import torch
 
@@ -138,6 +139,14 @@ def main() -> int:
     d_bhits = torch.zeros(max(len(brays), 1) * 16, dtype=torch.uint8, device="cuda")
     n_p, n_b = len(prays), len(brays)
     print(f"[bench] {n_p} primary rays, {n_b} bounce rays", file=sys.stderr, flush=True)
+
+    tuned = {}
+    if not args.no_autotune:  # launch shapes from the native autotuner (untimed setup)
+        cfg_p, ms_p = ctx.autotune(p_primary, d_prays.data_ptr(), n_p, d_phits.data_ptr(), sptr)
+        cfg_b, ms_b = ctx.autotune(p_bounce, d_brays.data_ptr(), n_b, d_bhits.data_ptr(), sptr)
+        tuned = {"primary": cfg_p, "bounce": cfg_b}
+        print(f"[bench] autotuned: primary {cfg_p} ({ms_p:.3f} ms), bounce {cfg_b} ({ms_b:.3f} ms)",
+              file=sys.stderr, flush=True)
 
     def step(evs=None):
         if evs is not None:
@@ -199,7 +208,8 @@ def main() -> int:
             "bounce_grays": round(n_b / (ms_b * 1e-3) / 1e9, 4),
             "ms_primary": round(ms_p, 4),
             "ms_bounce": round(ms_b, 4),
-            "launch": ctx.launch_config(),
+            "launch": {"primary": ctx.launch_config(0), "bounce": ctx.launch_config(2),
+                       "autotuned": bool(tuned)},
             "parallelism": "single GPU",
         },
     }

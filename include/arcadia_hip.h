@@ -178,8 +178,19 @@ typedef struct arc_launch_config {
   uint32_t refill_lanes;    /* a wave refills when at least this many lanes are idle (1..64) */
   uint32_t batch_rays;      /* rays claimed per queue fetch (per wave) */
 } arc_launch_config;
+/* Set the launch shape of every ray class (lod_bias 0, 1, >=2) / read the class-0 shape. */
 int arc_set_launch_config(arc_ctx* ctx, const arc_launch_config* cfg);
 int arc_get_launch_config(const arc_ctx* ctx, arc_launch_config* cfg);
+/* Launch shape of one ray class (lod_class 0, 1, 2 = the lod_bias classes of the params). */
+int arc_set_launch_config_class(arc_ctx* ctx, uint32_t lod_class, const arc_launch_config* cfg);
+int arc_get_launch_config_class(const arc_ctx* ctx, uint32_t lod_class, arc_launch_config* cfg);
+/* Autotuner (no reference counterpart; the reference dispatches fixed 8x8 workgroups): times
+ * candidate launch shapes of the persistent kernel on this ray set with HIP events on
+ * `stream`, keeps the fastest for the ray class of `params` and returns it (and its time in
+ * ms, median of `reps` launches). d_hits receives the last traced results. Blocking. */
+int arc_autotune(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n,
+                 arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
+                 float* best_ms);
 
 #ifdef __cplusplus
 }

@@ -152,3 +152,23 @@ def test_launch_shapes_do_not_change_results(ctx, cfg):
         _check(ctx, pools, A.TraceParams.exact(), rays, f"launch {cfg}")
     finally:
         ctx.set_launch_config()
+
+
+def test_autotuner_keeps_results_bit_exact(ctx):
+    torch = pytest.importorskip("torch")
+    pools = W.caves_world(1)
+    rays = W.random_rays(60000, (2, 2, 2), seed=31, inside=False)
+    params = A.TraceParams.exact(2, ignore_water=True)
+    ctx.upload(*pools)
+    d_rays = torch.from_numpy(rays.view(np.uint8)).cuda()
+    d_hits = torch.zeros(len(rays) * 16, dtype=torch.uint8, device="cuda")
+    try:
+        cfg, ms = ctx.autotune(params, d_rays.data_ptr(), len(rays), d_hits.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream, reps=1)
+        assert ms > 0 and cfg["block_threads"] in (128, 256)
+        assert ctx.launch_config(2) == cfg
+        got = ctx.trace(params, rays)
+    finally:
+        ctx.set_launch_config()
+    ref, _, _ = W.oracle_hits(pools, params, rays)
+    assert len(W.diff_records(got, ref)) == 0
