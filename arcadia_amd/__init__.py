@@ -17,6 +17,7 @@ from .native import (  # noqa: F401
     RESULT_DTYPE,
     ArcError,
     Context,
+    GbufferParams,
     HostWorld,
     TraceParams,
     bounce_rays,

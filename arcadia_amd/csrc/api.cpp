@@ -11,6 +11,8 @@
 #include <vector>
 
 #include "../../include/arcadia_hip.h"
+#include "../../include/arcadia_world.h"
+#include "gbuffer.h"
 #include "trace_launch.h"
 
 struct arc_ctx {
@@ -32,6 +34,11 @@ struct arc_ctx {
   arc_hit* d_hits = nullptr;
   uint64_t scratch_n = 0;
   arc_launch_config cfg[3]{};  // per ray class (lod_bias 0, 1, >=2)
+  // primary G-buffer pass: material palette and per-pass scratch
+  float* d_palette = nullptr;
+  arc_ray* d_gb_rays = nullptr;
+  arc_trace_result* d_gb_res = nullptr;
+  uint64_t gb_n = 0;
 };
 
 namespace {
@@ -142,7 +149,16 @@ int arc_create(int device, arc_ctx** out) {
     return ARC_ERR_HIP;
   }
   if (hipMemset(c->d_queue, 0, 64 * sizeof(uint32_t)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-    hipFree(c->d_queue);
+    (void)hipFree(c->d_queue);
+    delete c;
+    return ARC_ERR_HIP;
+  }
+  float pal[256 * 8];
+  for (uint32_t m = 0; m < 256; ++m) arc_palette_material(m, pal + 8 * m);
+  if (hipMalloc(&c->d_palette, sizeof pal) != hipSuccess ||
+      hipMemcpy(c->d_palette, pal, sizeof pal, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(c->d_queue);
+    (void)hipFree(c->d_palette);
     delete c;
     return ARC_ERR_HIP;
   }
@@ -154,12 +170,15 @@ void arc_destroy(arc_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipDeviceSynchronize();
-  hipFree(c->d_roots);
-  hipFree(c->d_nodes);
-  hipFree(c->d_bricks);
-  hipFree(c->d_queue);
-  hipFree(c->d_rays);
-  hipFree(c->d_hits);
+  (void)hipFree(c->d_roots);
+  (void)hipFree(c->d_nodes);
+  (void)hipFree(c->d_bricks);
+  (void)hipFree(c->d_queue);
+  (void)hipFree(c->d_rays);
+  (void)hipFree(c->d_hits);
+  (void)hipFree(c->d_palette);
+  (void)hipFree(c->d_gb_rays);
+  (void)hipFree(c->d_gb_res);
   delete c;
 }
 
@@ -232,9 +251,9 @@ int arc_world_upload(arc_ctx* c, const uint32_t grid[3], const uint32_t* roots, 
   }
   HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipDeviceSynchronize());
-  hipFree(c->d_roots);
-  hipFree(c->d_nodes);
-  hipFree(c->d_bricks);
+  (void)hipFree(c->d_roots);
+  (void)hipFree(c->d_nodes);
+  (void)hipFree(c->d_bricks);
   c->d_roots = nullptr;
   c->d_nodes = nullptr;
   c->d_bricks = nullptr;
@@ -291,7 +310,7 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
     HIP_TRY(c, hipMemcpy(nn, c->d_nodes, c->node_cap * sizeof(arc_node64), hipMemcpyDeviceToDevice));
     HIP_TRY(c, hipMemset(nn + c->node_cap, 0, (cap - c->node_cap) * sizeof(arc_node64)));
     HIP_TRY(c, hipDeviceSynchronize());
-    hipFree(c->d_nodes);
+    (void)hipFree(c->d_nodes);
     c->d_nodes = nn;
     c->node_cap = cap;
   }
@@ -302,7 +321,7 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
     HIP_TRY(c, hipMemcpy(nb, c->d_bricks, c->brick_cap * sizeof(arc_brick64), hipMemcpyDeviceToDevice));
     HIP_TRY(c, hipMemset(nb + c->brick_cap, 0, (cap - c->brick_cap) * sizeof(arc_brick64)));
     HIP_TRY(c, hipDeviceSynchronize());
-    hipFree(c->d_bricks);
+    (void)hipFree(c->d_bricks);
     c->d_bricks = nb;
     c->brick_cap = cap;
   }
@@ -427,14 +446,61 @@ int arc_autotune(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_ra
   return ARC_OK;
 }
 
+int arc_primary_gbuffer(arc_ctx* c, const arc_trace_params* params, const arc_gbuffer_params* g, uint32_t* d_depth_mat,
+                        uint32_t* d_norm_motion, uint32_t* d_albedo_ao, void* stream) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!g || !params || !d_depth_mat || !d_norm_motion || !d_albedo_ao)
+    return fail(c, ARC_ERR_INVALID, "arc_primary_gbuffer: null argument");
+  if (g->width == 0 || g->height == 0 || g->width > 16384 || g->height > 16384)
+    return fail(c, ARC_ERR_INVALID, "image size must be 1..16384 per axis");
+  if (params->semantics != ARC_SEM_WORLD) return fail(c, ARC_ERR_INVALID, "the G-buffer pass traces world semantics");
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "trace before arc_world_upload");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const uint64_t tiles = (uint64_t)((g->width + 7) / 8) * ((g->height + 7) / 8);
+  const uint64_t n = tiles * 64u;
+  if (n > c->gb_n) {
+    HIP_TRY(c, hipStreamSynchronize((hipStream_t)stream));
+    (void)hipFree(c->d_gb_rays);
+    (void)hipFree(c->d_gb_res);
+    c->d_gb_rays = nullptr;
+    c->d_gb_res = nullptr;
+    c->gb_n = 0;
+    HIP_TRY(c, hipMalloc(&c->d_gb_rays, n * sizeof(arc_ray)));
+    HIP_TRY(c, hipMalloc(&c->d_gb_res, n * sizeof(arc_trace_result)));
+    c->gb_n = n;
+  }
+  arcdev::GbufArgs a;
+  std::memcpy(a.eye, g->eye, sizeof a.eye);
+  std::memcpy(a.forward, g->forward, sizeof a.forward);
+  std::memcpy(a.right, g->right, sizeof a.right);
+  std::memcpy(a.up, g->up, sizeof a.up);
+  a.tan_half_fov = g->tan_half_fov;
+  a.aspect = g->aspect;
+  a.jitter[0] = g->jitter[0];
+  a.jitter[1] = g->jitter[1];
+  std::memcpy(a.prev_view_proj, g->prev_view_proj, sizeof a.prev_view_proj);
+  a.width = g->width;
+  a.height = g->height;
+  a.max_dist_m = g->max_dist_m;
+  a.palette = c->d_palette;
+  hipError_t e = arcdev::launch_gbuffer_raygen(a, c->d_gb_rays, (uint32_t)n, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "raygen launch: %s", hipGetErrorString(e));
+  int rc = launch(c, params, c->d_gb_rays, n, c->d_gb_res, true, stream);
+  if (rc != ARC_OK) return rc;
+  e = arcdev::launch_gbuffer_pack(a, c->d_gb_rays, c->d_gb_res, (uint32_t)n, d_depth_mat, d_norm_motion, d_albedo_ao,
+                                  (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "G-buffer pack launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
 int arc_trace(arc_ctx* c, const arc_trace_params* params, const arc_ray* rays, uint64_t n, arc_hit* hits_out) {
   if (!c) return ARC_ERR_INVALID;
   if (n == 0) return ARC_OK;
   if (!rays || !hits_out) return fail(c, ARC_ERR_INVALID, "null ray or hit pointer");
   HIP_TRY(c, hipSetDevice(c->device));
   if (n > c->scratch_n) {
-    hipFree(c->d_rays);
-    hipFree(c->d_hits);
+    (void)hipFree(c->d_rays);
+    (void)hipFree(c->d_hits);
     c->d_rays = nullptr;
     c->d_hits = nullptr;
     c->scratch_n = 0;

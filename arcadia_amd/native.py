@@ -108,6 +108,26 @@ class GpuPatch(C.Structure):
                 ("n_bricks", C.c_uint32), ("bricks", C.c_void_p)]
 
 
+class GbufferParams(C.Structure):
+    """arc_gbuffer_params: camera basis, jitter, previous view-projection (column-major)."""
+    _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3),
+                ("up", C.c_float * 3), ("tan_half_fov", C.c_float), ("aspect", C.c_float),
+                ("jitter", C.c_float * 2), ("prev_view_proj", C.c_float * 16), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("max_dist_m", C.c_float), ("reserved", C.c_uint32)]
+
+    @classmethod
+    def from_camera(cls, cam: "Camera", width: int, height: int, prev_view_proj=None,
+                    max_dist_m: float = 250.0) -> "GbufferParams":
+        g = cls()
+        for f in ("eye", "forward", "right", "up", "jitter"):
+            setattr(g, f, getattr(cam, f))
+        g.tan_half_fov, g.aspect = cam.tan_half_fov, cam.aspect
+        m = np.eye(4, dtype=np.float32) if prev_view_proj is None else np.asarray(prev_view_proj, np.float32)
+        g.prev_view_proj = (C.c_float * 16)(*m.reshape(-1).tolist())
+        g.width, g.height, g.max_dist_m = width, height, max_dist_m
+        return g
+
+
 _lib = None
 
 # (name, restype, argtypes)
@@ -126,6 +146,8 @@ _SIGS = [
                                    C.c_void_p]),
     ("arc_trace_full_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
                                         C.c_void_p]),
+    ("arc_primary_gbuffer", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(GbufferParams), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
@@ -425,3 +447,9 @@ class Context:
     def trace_full_device(self, params: TraceParams, d_rays: int, n: int, d_out: int, stream: int = 0) -> None:
         self._check(lib().arc_trace_full_device(self._h, C.byref(params), d_rays, n, d_out, stream or None),
                     "arc_trace_full_device")
+
+    def primary_gbuffer(self, params: TraceParams, g: GbufferParams, d_depth_mat: int, d_norm_motion: int,
+                        d_albedo_ao: int, stream: int = 0) -> None:
+        """Primary G-buffer pass into device buffers of 2, 2 and 1 u32 per pixel (asynchronous)."""
+        self._check(lib().arc_primary_gbuffer(self._h, C.byref(params), C.byref(g), d_depth_mat, d_norm_motion,
+                                              d_albedo_ao, stream or None), "arc_primary_gbuffer")

@@ -171,6 +171,31 @@ int arc_trace_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray
 /* Full TraceResult records (hit position, normal, lod rgb, voxel hash, contact AO). */
 int arc_trace_full_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
                           uint64_t n, arc_trace_result* d_out, void* stream);
+/* ---- primary G-buffer pass -------------------------------------------------------------- */
+/* Camera and frame inputs of primary_raymarch (the CameraUniforms fields it reads,
+ * shaders/common/types.wgsl:19-63), 144 B. prev_view_proj is column-major. */
+typedef struct arc_gbuffer_params {
+  float eye[3], forward[3], right[3], up[3];
+  float tan_half_fov;  /* tan of half the vertical fov */
+  float aspect;        /* width / height */
+  float jitter[2];     /* sub-pixel jitter in [-0.5, 0.5] */
+  float prev_view_proj[16];
+  uint32_t width, height;
+  float max_dist_m;    /* 250 m in the reference */
+  uint32_t reserved;
+} arc_gbuffer_params;
+
+/* The opaque-world path of record_primary_raymarch_pass / primary_raymarch
+ * (crates/arcadia-render/src/pipeline.rs:150-166, shaders/primary_gbuffer.wgsl:93-312): one
+ * pinhole ray per pixel traced with `params` (lod_bias 0), written as the 20 B/px G-buffer:
+ * depth_mat 2 u32/px (depth f32 bits, mat | flags << 16), norm_motion 2 u32/px
+ * (pack2x16snorm(oct(normal)), pack2x16float(motion)), albedo_ao 1 u32/px (pack4x8unorm).
+ * Device pointers, asynchronous on `stream`. Water/mirror secondary rays and rigid bodies are not
+ * traced (their pixels take the opaque branch). */
+int arc_primary_gbuffer(arc_ctx* ctx, const arc_trace_params* params, const arc_gbuffer_params* g,
+                        uint32_t* d_depth_mat, uint32_t* d_norm_motion, uint32_t* d_albedo_ao,
+                        void* stream);
+
 /* Kernel launch shape of the persistent ray-queue kernel; 0 fields = autotuned defaults. */
 typedef struct arc_launch_config {
   uint32_t blocks_per_cu;   /* resident workgroups per compute unit */

@@ -966,3 +966,170 @@ void orc_brute_dense(const uint8_t* dense, const uint32_t dims[3], const arc_ray
   for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
   pthread_mutex_destroy(&j.mu);
 }
+
+/* ---- primary G-buffer: shaders/primary_gbuffer.wgsl:93-312 (opaque path) -------------------- */
+uint16_t orc_f32_to_f16(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  uint32_t sign = (u >> 16) & 0x8000u;
+  uint32_t e = (u >> 23) & 0xFFu;
+  uint32_t m = u & 0x7FFFFFu;
+  if (e == 0xFFu) return (uint16_t)(sign | 0x7C00u | (m ? 0x200u | (m >> 13) : 0u)); /* inf / NaN */
+  int32_t he = (int32_t)e - 127 + 15;
+  if (he >= 31) return (uint16_t)(sign | 0x7C00u); /* overflow -> inf */
+  if (he <= 0) {                                   /* subnormal half (or zero) */
+    if (he < -10) return (uint16_t)sign;
+    uint32_t mant = m | 0x800000u;
+    uint32_t shift = (uint32_t)(14 - he);
+    uint32_t h = mant >> shift;
+    uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+  }
+  uint32_t h = ((uint32_t)he << 10) | (m >> 13);
+  uint32_t rem = m & 0x1FFFu;
+  if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++; /* may carry into the exponent: correct */
+  return (uint16_t)(sign | h);
+}
+
+static inline uint32_t pk_snorm16(float e) {
+  float c = fminf(1.0f, fmaxf(-1.0f, e));
+  float s = 32767.0f * c;
+  return (uint32_t)(int32_t)floorf(0.5f + s) & 0xFFFFu;
+}
+static inline uint32_t pk_unorm8(float e) {
+  float c = fminf(1.0f, fmaxf(0.0f, e));
+  float s = 255.0f * c;
+  return (uint32_t)floorf(0.5f + s);
+}
+/* oct_encode (types.wgsl:112-119) + pack_norm_motion (:310-313) */
+static void pk_norm_motion(const float n[3], float mx, float my, uint32_t out[2]) {
+  float l1 = fabsf(n[0]) + fabsf(n[1]);
+  l1 = l1 + fabsf(n[2]);
+  float inv = 1.0f / fmaxf(l1, 1e-6f);
+  float px = n[0] * inv, py = n[1] * inv;
+  if (n[2] < 0.0f) {
+    float wx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+    float wy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+    px = wx;
+    py = wy;
+  }
+  out[0] = pk_snorm16(px) | (pk_snorm16(py) << 16);
+  out[1] = (uint32_t)orc_f32_to_f16(mx) | ((uint32_t)orc_f32_to_f16(my) << 16);
+}
+
+typedef struct gb_job {
+  const orc_world* w;
+  const arc_trace_params* p;
+  const arc_gbuffer_params* g;
+  const float* pal;
+  uint32_t *dm, *nm, *aa;
+  uint32_t n;
+  uint32_t next;
+  pthread_mutex_t mu;
+} gb_job;
+
+static void gb_one(gb_job* j, uint32_t i) {
+  const arc_gbuffer_params* g = j->g;
+  uint32_t tiles_x = (g->width + 7u) / 8u, tile = i / 64u, k = i % 64u;
+  uint32_t px = (tile % tiles_x) * 8u + (k % 8u), py = (tile / tiles_x) * 8u + (k / 8u);
+  if (px >= g->width || py >= g->height) return;
+  /* primary_gbuffer.wgsl:103-115 */
+  float ux = ((float)px + 0.5f) + g->jitter[0];
+  float uy = ((float)py + 0.5f) + g->jitter[1];
+  float uvx = ux / (float)g->width, uvy = uy / (float)g->height;
+  float ndx = uvx * 2.0f - 1.0f, ndy = 1.0f - uvy * 2.0f;
+  float sx = (ndx * g->aspect) * g->tan_half_fov, sy = ndy * g->tan_half_fov;
+  float v[3];
+  for (int a = 0; a < 3; ++a) {
+    float rr = g->right[a] * sx, uu = g->up[a] * sy;
+    v[a] = (g->forward[a] + rr) + uu;
+  }
+  float dd = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+  float len = sqrtf(dd);
+  arc_ray ray = {g->eye[0], g->eye[1], g->eye[2], 0.0f, v[0] / len, v[1] / len, v[2] / len, g->max_dist_m};
+  arc_trace_result r;
+  trace_world_one(j->w, j->p, &ray, &r, NULL);
+  uint32_t pix = py * g->width + px;
+  if (!(r.hit & 1u)) {
+    float d3[3] = {ray.dx, ray.dy, ray.dz};
+    uint32_t nmv[2];
+    pk_norm_motion(d3, 0.0f, 0.0f, nmv);
+    float far = 1e5f;
+    memcpy(&j->dm[2 * pix], &far, 4);
+    j->dm[2 * pix + 1] = 0u;
+    j->nm[2 * pix] = nmv[0];
+    j->nm[2 * pix + 1] = nmv[1];
+    j->aa[pix] = pk_unorm8(1.0f) << 24;
+    return;
+  }
+  float hit_dist = r.t_voxels * 0.0625f;
+  uint32_t mat = r.mat & 0xFFu;
+  const float* m = j->pal + 8u * mat;
+  float tt = clampf_((hit_dist - 18.0f) / (48.0f - 18.0f), 0.0f, 1.0f);
+  float sm = tt * tt * (3.0f - 2.0f * tt);
+  float wgt = 1.0f - sm;
+  float v_rand = (((float)(r.voxel_hash & 0xFFu) / 255.0f) * 2.0f - 1.0f) * wgt;
+  float kk = 1.0f + v_rand * m[7];
+  float alb[3];
+  for (int a = 0; a < 3; ++a) alb[a] = clampf_(m[a] * kk, 0.01f, 1.0f);
+  float pu = ((float)px + 0.5f) / (float)g->width, pv = ((float)py + 0.5f) / (float)g->height;
+  float clip[4];
+  for (int row = 0; row < 4; ++row) {
+    float a0 = g->prev_view_proj[0 * 4 + row] * r.hit_pos_m[0];
+    float a1 = g->prev_view_proj[1 * 4 + row] * r.hit_pos_m[1];
+    float a2 = g->prev_view_proj[2 * 4 + row] * r.hit_pos_m[2];
+    float a3 = g->prev_view_proj[3 * 4 + row] * 1.0f;
+    clip[row] = ((a0 + a1) + a2) + a3;
+  }
+  float mx = -2.0f, my = -2.0f;
+  if (clip[3] > 0.001f) {
+    float nx = clip[0] / clip[3], ny = clip[1] / clip[3];
+    mx = (nx * 0.5f + 0.5f) - pu;
+    my = (0.5f - ny * 0.5f) - pv;
+  }
+  memcpy(&j->dm[2 * pix], &hit_dist, 4);
+  j->dm[2 * pix + 1] = mat & 0xFFFFu;
+  uint32_t nmv[2];
+  pk_norm_motion(r.normal, mx, my, nmv);
+  j->nm[2 * pix] = nmv[0];
+  j->nm[2 * pix + 1] = nmv[1];
+  float ao = clampf_(r.contact_ao, 0.0f, 1.0f);
+  j->aa[pix] = pk_unorm8(alb[0]) | (pk_unorm8(alb[1]) << 8) | (pk_unorm8(alb[2]) << 16) | (pk_unorm8(ao) << 24);
+}
+
+static void* gb_worker(void* arg) {
+  gb_job* j = (gb_job*)arg;
+  for (;;) {
+    pthread_mutex_lock(&j->mu);
+    uint32_t s = j->next;
+    j->next += 256u;
+    pthread_mutex_unlock(&j->mu);
+    if (s >= j->n) break;
+    uint32_t e = s + 256u < j->n ? s + 256u : j->n;
+    for (uint32_t i = s; i < e; ++i) gb_one(j, i);
+  }
+  return NULL;
+}
+
+void orc_primary_gbuffer(const orc_world* w, const arc_trace_params* p, const arc_gbuffer_params* g,
+                         const float* palette, uint32_t* depth_mat, uint32_t* norm_motion,
+                         uint32_t* albedo_ao, int nthreads) {
+  gb_job j;
+  j.w = w;
+  j.p = p;
+  j.g = g;
+  j.pal = palette;
+  j.dm = depth_mat;
+  j.nm = norm_motion;
+  j.aa = albedo_ao;
+  j.n = ((g->width + 7u) / 8u) * ((g->height + 7u) / 8u) * 64u;
+  j.next = 0;
+  pthread_mutex_init(&j.mu, NULL);
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 64) nthreads = 64;
+  pthread_t th[64];
+  for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, gb_worker, &j);
+  for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+  pthread_mutex_destroy(&j.mu);
+}

@@ -62,6 +62,15 @@ void orc_trace_meters(const orc_world* w, const arc_ray* rays, uint64_t n, arc_t
 /* Compact records as the kernels write them. */
 void orc_compact(const arc_trace_result* full, uint64_t n, int world_units, arc_hit* out);
 
+/* primary_raymarch's opaque path (shaders/primary_gbuffer.wgsl:93-312): rays in padded 8x8-tile
+ * order, traced with orc_trace_world, packed into depth_mat (2 u32/px), norm_motion (2 u32/px)
+ * and albedo_ao (1 u32/px). `palette` = 256 x {albedo rgb, roughness, emission rgb, variance}. */
+void orc_primary_gbuffer(const orc_world* w, const arc_trace_params* p, const arc_gbuffer_params* g,
+                         const float* palette, uint32_t* depth_mat, uint32_t* norm_motion,
+                         uint32_t* albedo_ao, int nthreads);
+/* f32 -> f16 bits, round to nearest even (the conversion of pack2x16float). */
+uint16_t orc_f32_to_f16(float x);
+
 uint32_t orc_query_voxel(const orc_world* w, uint32_t x, uint32_t y, uint32_t z);
 /* Dense material grid (x fastest) of the whole world, size (gx*256)*(gy*256)*(gz*256). */
 void orc_decode_dense(const orc_world* w, uint8_t* dense, int nthreads);

@@ -64,6 +64,10 @@ def lib() -> C.CDLL:
         h.orc_query_voxel.restype = C.c_uint32
         h.orc_decode_dense.argtypes = [P, P, C.c_int]
         h.orc_brute_dense.argtypes = [P, P, P, C.c_uint64, C.c_int, C.c_double, P, C.c_int]
+        h.orc_primary_gbuffer.argtypes = [P, P, P, P, P, P, P, C.c_int]
+        h.orc_primary_gbuffer.restype = None
+        h.orc_f32_to_f16.argtypes = [C.c_float]
+        h.orc_f32_to_f16.restype = C.c_uint16
         for f in (h.orc_trace_world, h.orc_trace_chunk, h.orc_trace_meters, h.orc_compact, h.orc_decode_dense,
                   h.orc_brute_dense):
             f.restype = None
@@ -160,3 +164,23 @@ def brute(dense: np.ndarray, rays: np.ndarray, world_units: bool = True, max_t_s
         lib().orc_brute_dense(dense.ctypes.data, dims, rays.ctypes.data, len(rays), int(world_units), max_t_scale,
                               out.ctypes.data, _threads(nthreads))
     return out
+
+
+def primary_gbuffer(world: World, params, g, palette: np.ndarray, nthreads: int = 0):
+    """primary_raymarch opaque path -> (depth_mat (H,W,2) u32, norm_motion (H,W,2) u32, albedo_ao (H,W) u32).
+
+    `g` is an arc_gbuffer_params ctypes structure, `palette` the 256x8 float32 material table."""
+    W, H = int(g.width), int(g.height)
+    dm = np.zeros((H, W, 2), np.uint32)
+    nm = np.zeros((H, W, 2), np.uint32)
+    aa = np.zeros((H, W), np.uint32)
+    pal = np.ascontiguousarray(palette, np.float32)
+    assert pal.shape == (256, 8)
+    q = _params(params)
+    lib().orc_primary_gbuffer(world.ref, C.byref(q), C.byref(g), pal.ctypes.data, dm.ctypes.data, nm.ctypes.data,
+                              aa.ctypes.data, _threads(nthreads))
+    return dm, nm, aa
+
+
+def f32_to_f16_bits(x: float) -> int:
+    return int(lib().orc_f32_to_f16(float(x)))
