@@ -13,6 +13,8 @@
 #include "../../include/arcadia_hip.h"
 #include "../../include/arcadia_world.h"
 #include "gbuffer.h"
+#include "patch.h"
+#include "patch_plan.h"
 #include "trace_launch.h"
 
 struct arc_ctx {
@@ -39,6 +41,10 @@ struct arc_ctx {
   arc_ray* d_gb_rays = nullptr;
   arc_trace_result* d_gb_res = nullptr;
   uint64_t gb_n = 0;
+  // arc_world_patch staging (pinned host + device copy of the same size)
+  void* h_stage = nullptr;
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
 };
 
 namespace {
@@ -168,8 +174,8 @@ int arc_create(int device, arc_ctx** out) {
 
 void arc_destroy(arc_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  hipDeviceSynchronize();
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
   (void)hipFree(c->d_roots);
   (void)hipFree(c->d_nodes);
   (void)hipFree(c->d_bricks);
@@ -179,6 +185,8 @@ void arc_destroy(arc_ctx* c) {
   (void)hipFree(c->d_palette);
   (void)hipFree(c->d_gb_rays);
   (void)hipFree(c->d_gb_res);
+  (void)hipFree(c->d_stage);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   delete c;
 }
 
@@ -286,25 +294,41 @@ int arc_world_device_pools(const arc_ctx* c, const uint32_t** roots, const arc_n
   return ARC_OK;
 }
 
+int arc_world_read_pools(arc_ctx* c, uint32_t* roots, arc_node64* nodes, uint64_t node_first, uint64_t n_nodes,
+                         arc_brick64* bricks, uint64_t brick_first, uint64_t n_bricks) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "arc_world_read_pools before arc_world_upload");
+  if ((n_nodes && !nodes) || (n_bricks && !bricks)) return fail(c, ARC_ERR_INVALID, "null output");
+  if (node_first + n_nodes > c->node_cap || brick_first + n_bricks > c->brick_cap)
+    return fail(c, ARC_ERR_INVALID, "read range outside the pools");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipDeviceSynchronize());
+  const uint64_t nch = (uint64_t)c->grid[0] * c->grid[1] * c->grid[2];
+  if (roots) HIP_TRY(c, hipMemcpy(roots, c->d_roots, nch * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (n_nodes)
+    HIP_TRY(c, hipMemcpy(nodes, c->d_nodes + node_first, n_nodes * sizeof(arc_node64), hipMemcpyDeviceToHost));
+  if (n_bricks)
+    HIP_TRY(c, hipMemcpy(bricks, c->d_bricks + brick_first, n_bricks * sizeof(arc_brick64), hipMemcpyDeviceToHost));
+  return ARC_OK;
+}
+
 int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
   if (!c) return ARC_ERR_INVALID;
   if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "arc_world_patch before arc_world_upload");
-  if (n && !patches) return fail(c, ARC_ERR_INVALID, "null patches");
-  HIP_TRY(c, hipSetDevice(c->device));
+  if (n == 0) return ARC_OK;
+  if (!patches) return fail(c, ARC_ERR_INVALID, "null patches");
   const uint64_t nchunks = (uint64_t)c->grid[0] * c->grid[1] * c->grid[2];
-  // Grow the pools first (renderer.rs:156-223: copy old contents into a larger buffer).
-  uint64_t need_nodes = c->node_cap, need_bricks = c->brick_cap;
-  for (uint64_t i = 0; i < n; ++i) {
-    const arc_gpu_patch& p = patches[i];
-    if (p.chunk_flat_idx >= nchunks) return fail(c, ARC_ERR_INVALID, "patch %llu: chunk index out of range", (unsigned long long)i);
-    if ((p.n_nodes && !p.nodes) || (p.n_bricks && !p.bricks)) return fail(c, ARC_ERR_INVALID, "patch %llu: null slice", (unsigned long long)i);
-    uint64_t ne = (uint64_t)p.node_offset + p.n_nodes, be = (uint64_t)p.brick_offset + p.n_bricks;
-    if (ne > ARC_PTR_INDEX_MASK || be > ARC_PTR_INDEX_MASK) return fail(c, ARC_ERR_CAPACITY, "patch %llu beyond the 30-bit pointer range", (unsigned long long)i);
-    if (ne > need_nodes) need_nodes = ne;
-    if (be > need_bricks) need_bricks = be;
-  }
-  if (need_nodes > c->node_cap) {
-    uint64_t cap = need_nodes + (need_nodes >> 2) + 1024;
+  arcplan::Plan plan;
+  std::string err;
+  const int rc = arcplan::plan_patches(patches, n, nchunks, c->node_cap, c->brick_cap, plan, err);
+  if (rc != ARC_OK) return fail(c, rc, "%s", err.c_str());
+  HIP_TRY(c, hipSetDevice(c->device));
+  // Work queued on any stream may still read the pools.
+  HIP_TRY(c, hipDeviceSynchronize());
+  // Grow a pool that a patch writes past (renderer.rs:156-223: copy into a larger buffer).
+  if (plan.need_nodes > c->node_cap) {
+    const uint64_t cap =
+        std::min<uint64_t>(plan.need_nodes + (plan.need_nodes >> 2) + 1024, (uint64_t)ARC_PTR_INDEX_MASK + 1);
     arc_node64* nn = nullptr;
     HIP_TRY(c, hipMalloc(&nn, cap * sizeof(arc_node64)));
     HIP_TRY(c, hipMemcpy(nn, c->d_nodes, c->node_cap * sizeof(arc_node64), hipMemcpyDeviceToDevice));
@@ -314,8 +338,9 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
     c->d_nodes = nn;
     c->node_cap = cap;
   }
-  if (need_bricks > c->brick_cap) {
-    uint64_t cap = need_bricks + (need_bricks >> 2) + 1024;
+  if (plan.need_bricks > c->brick_cap) {
+    const uint64_t cap =
+        std::min<uint64_t>(plan.need_bricks + (plan.need_bricks >> 2) + 1024, (uint64_t)ARC_PTR_INDEX_MASK + 1);
     arc_brick64* nb = nullptr;
     HIP_TRY(c, hipMalloc(&nb, cap * sizeof(arc_brick64)));
     HIP_TRY(c, hipMemcpy(nb, c->d_bricks, c->brick_cap * sizeof(arc_brick64), hipMemcpyDeviceToDevice));
@@ -325,18 +350,29 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
     c->d_bricks = nb;
     c->brick_cap = cap;
   }
-  for (uint64_t i = 0; i < n; ++i) {
-    const arc_gpu_patch& p = patches[i];
-    if (p.n_nodes)
-      HIP_TRY(c, hipMemcpy(c->d_nodes + p.node_offset, p.nodes, (size_t)p.n_nodes * sizeof(arc_node64), hipMemcpyHostToDevice));
-    if (p.n_bricks)
-      HIP_TRY(c, hipMemcpy(c->d_bricks + p.brick_offset, p.bricks, (size_t)p.n_bricks * sizeof(arc_brick64), hipMemcpyHostToDevice));
-    HIP_TRY(c, hipMemcpy(c->d_roots + p.chunk_flat_idx, &p.new_root_node_idx, sizeof(uint32_t), hipMemcpyHostToDevice));
-    uint64_t ne = (uint64_t)p.node_offset + p.n_nodes, be = (uint64_t)p.brick_offset + p.n_bricks;
-    if (ne > c->n_nodes) c->n_nodes = ne;
-    if (be > c->n_bricks) c->n_bricks = be;
+  const size_t bytes = plan.total_units * 16;
+  if (bytes > c->stage_bytes) {
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->d_stage) (void)hipFree(c->d_stage);
+    c->h_stage = nullptr;
+    c->d_stage = nullptr;
+    c->stage_bytes = 0;
+    const size_t cap = bytes + bytes / 4 + 65536;
+    HIP_TRY(c, hipHostMalloc(&c->h_stage, cap, hipHostMallocDefault));
+    HIP_TRY(c, hipMalloc(&c->d_stage, cap));
+    c->stage_bytes = cap;
   }
+  arcplan::fill_staging(plan, c->h_stage);
+  HIP_TRY(c, hipMemcpy(c->d_stage, c->h_stage, bytes, hipMemcpyHostToDevice));
+  const uint4* d = static_cast<const uint4*>(c->d_stage);
+  const uint32_t np = (uint32_t)plan.pieces.size();
+  hipError_t e = arcdev::launch_apply_patches(reinterpret_cast<const arcdev::PatchPiece*>(d), np,
+                                              reinterpret_cast<const uint2*>(d + np), (uint32_t)plan.roots.size(), d,
+                                              c->d_nodes, c->d_bricks, c->d_roots, nullptr);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "patch scatter launch: %s", hipGetErrorString(e));
   HIP_TRY(c, hipDeviceSynchronize());
+  c->n_nodes = std::max(c->n_nodes, plan.max_node_written);
+  c->n_bricks = std::max(c->n_bricks, plan.max_brick_written);
   return ARC_OK;
 }
 

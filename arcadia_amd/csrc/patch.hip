@@ -1,0 +1,38 @@
+// Scatter kernel of arc_world_patch: copies staged node / brick slices into the device pools and
+// writes chunk roots, in one launch. The host resolves overlapping patches beforehand (the last
+// patch touching a pool index wins, as sequential buffer writes would), so pieces are disjoint and
+// blocks run in any order.
+#include <hip/hip_runtime.h>
+
+#include "patch.h"
+
+namespace arcdev {
+
+__global__ __launch_bounds__(256) void k_apply_patches(const PatchPiece* __restrict__ pieces, uint32_t n_pieces,
+                                                       const uint2* __restrict__ root_writes, uint32_t n_roots,
+                                                       const uint4* __restrict__ src, uint4* __restrict__ nodes,
+                                                       uint4* __restrict__ bricks, uint32_t* __restrict__ roots) {
+  const uint32_t b = blockIdx.x;
+  if (b < n_pieces) {
+    const PatchPiece p = pieces[b];
+    uint4* dst = (p.pool == 0 ? nodes : bricks) + p.dst_unit;
+    const uint4* s = src + p.src_unit;
+    for (uint32_t i = threadIdx.x; i < p.n_units; i += blockDim.x) dst[i] = s[i];
+  } else {
+    for (uint32_t i = threadIdx.x; i < n_roots; i += blockDim.x) {
+      const uint2 r = root_writes[i];
+      roots[r.x] = r.y;
+    }
+  }
+}
+
+hipError_t launch_apply_patches(const PatchPiece* pieces, uint32_t n_pieces, const uint2* root_writes, uint32_t n_roots,
+                                const uint4* src, void* nodes, void* bricks, uint32_t* roots, hipStream_t s) {
+  const uint32_t grid = n_pieces + (n_roots ? 1u : 0u);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_patches, dim3(grid), dim3(256), 0, s, pieces, n_pieces, root_writes, n_roots, src,
+                     static_cast<uint4*>(nodes), static_cast<uint4*>(bricks), roots);
+  return hipGetLastError();
+}
+
+}  // namespace arcdev

@@ -304,6 +304,10 @@ void pack_world(World& w) {
   w.nodes.clear();
   w.bricks.clear();
   w.allocs.assign(nchunks, ChunkAlloc{0, 0, 0, 0});
+  w.free_node_spans.clear();
+  w.free_brick_spans.clear();
+  w.pending.clear();
+  w.views.clear();
   w.nodes.push_back(make_node(0, 0, 0));
   arc_brick64 zb;
   std::memset(&zb, 0, sizeof zb);

@@ -27,6 +27,10 @@ ARC_GEN_SPARSE = 2
 ARC_FACE_NONE = 255
 ARC_HIT_FLAG_HIT = 1
 ARC_HIT_FLAG_LOD = 2
+ARC_IS_LEAF_FLAG = 0x80000000
+ARC_IS_UNIFORM_FLAG = 0x40000000
+ARC_PTR_INDEX_MASK = 0x3FFFFFFF
+ARC_EMPTY_CHUNK_SENTINEL = 0xFFFFFFFF
 EMPTY_CHUNK_SENTINEL = 0xFFFFFFFF
 
 NODE_DTYPE = np.dtype([("mask_low", "<u4"), ("mask_high", "<u4"), ("child_ptr", "<u4"), ("lod_payload", "<u4")])
@@ -108,6 +112,19 @@ class GpuPatch(C.Structure):
                 ("n_bricks", C.c_uint32), ("bricks", C.c_void_p)]
 
 
+class EditReport(C.Structure):
+    """arc_edit_report (ChunkEditReport, set.rs:17-80)."""
+    _fields_ = [("min_dirty_node", C.c_uint64), ("max_dirty_node", C.c_uint64), ("min_dirty_brick", C.c_uint64),
+                ("max_dirty_brick", C.c_uint64), ("added_voxels", C.c_uint32), ("had_structural_growth", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+    def has_dirty_nodes(self) -> bool:
+        return self.min_dirty_node <= self.max_dirty_node
+
+    def has_dirty_bricks(self) -> bool:
+        return self.min_dirty_brick <= self.max_dirty_brick
+
+
 class GbufferParams(C.Structure):
     """arc_gbuffer_params: camera basis, jitter, previous view-projection (column-major)."""
     _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3),
@@ -141,6 +158,10 @@ _SIGS = [
     ("arc_world_patch", C.c_int, [C.c_void_p, C.POINTER(GpuPatch), C.c_uint64]),
     ("arc_world_device_pools", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                          C.POINTER(C.c_void_p)]),
+    ("arc_world_patch_host", C.c_int, [C.POINTER(GpuPatch), C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
+                                       C.c_uint64, C.c_void_p, C.c_uint64]),
+    ("arc_world_read_pools", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
+                                       C.c_uint64, C.c_uint64]),
     ("arc_trace", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p]),
     ("arc_trace_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
                                    C.c_void_p]),
@@ -168,6 +189,24 @@ _SIGS = [
     ("arc_palette_lod_payload", C.c_uint32, [C.c_uint32]),
     ("arc_world_generate", C.c_int, [C.c_void_p, C.POINTER(GenParams), C.c_int]),
     ("arc_gen_voxel", C.c_uint32, [C.c_void_p, C.POINTER(GenParams), C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("arc_chunk_set_voxels", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(EditReport)]),
+    ("arc_chunk_clear_voxels", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32),
+                                         C.POINTER(EditReport)]),
+    ("arc_chunk_carve_sphere", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.c_float, C.c_int,
+                                         C.POINTER(C.c_uint32), C.POINTER(EditReport)]),
+    ("arc_chunk_compact", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("arc_chunk_info", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    ("arc_world_host_voxel", C.c_uint32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
+    ("arc_world_host_sync_chunk", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("arc_world_host_sync_chunk_partial", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(EditReport)]),
+    ("arc_world_host_set_voxels", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32)]),
+    ("arc_world_host_clear_voxels", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32)]),
+    ("arc_world_host_carve_crater", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_float, C.c_int,
+                                              C.POINTER(C.c_uint32)]),
+    ("arc_world_host_pending_patches", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(GpuPatch)),
+                                                 C.POINTER(C.c_uint64)]),
+    ("arc_world_host_clear_patches", C.c_int, [C.c_void_p]),
     ("arc_camera_look", C.c_int, [C.POINTER(C.c_float), C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
                                   C.POINTER(Camera)]),
     ("arc_gen_primary_rays", C.c_int, [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_void_p,
@@ -298,6 +337,104 @@ class HostWorld:
         if rc != ARC_OK:
             raise ArcError(rc, "pack")
 
+    # ---- in-place edits (world_edit.cpp) ----
+    def _rc(self, rc, what):
+        if rc != ARC_OK:
+            raise ArcError(rc, what)
+
+    def set_voxels_local(self, idx: int, voxels) -> EditReport:
+        """ChunkTree64::set_voxels_local_tracked on chunk `idx`; voxels = (x, y, z, mat) rows."""
+        v = np.ascontiguousarray(np.asarray(voxels, dtype=np.uint32).reshape(-1, 4))
+        rep = EditReport()
+        self._rc(lib().arc_chunk_set_voxels(self._h, idx, _ptr(v) if v.size else None, v.shape[0], C.byref(rep)),
+                 "arc_chunk_set_voxels")
+        return rep
+
+    def clear_voxels_local(self, idx: int, voxels):
+        v = np.ascontiguousarray(np.asarray(voxels, dtype=np.uint32).reshape(-1, 3))
+        rep, k = EditReport(), C.c_uint32()
+        self._rc(lib().arc_chunk_clear_voxels(self._h, idx, _ptr(v) if v.size else None, v.shape[0], C.byref(k),
+                                              C.byref(rep)), "arc_chunk_clear_voxels")
+        return k.value, rep
+
+    def carve_sphere_local(self, idx: int, center, radius: float, scorch_rim: bool = False):
+        c = (C.c_float * 3)(*center)
+        d, rep = C.c_uint32(), EditReport()
+        self._rc(lib().arc_chunk_carve_sphere(self._h, idx, c, radius, int(scorch_rim), C.byref(d), C.byref(rep)),
+                 "arc_chunk_carve_sphere")
+        return d.value, rep
+
+    def compact_chunk(self, idx: int) -> None:
+        self._rc(lib().arc_chunk_compact(self._h, idx), "arc_chunk_compact")
+
+    def chunk_info(self, idx: int) -> dict:
+        nn, nb, sv = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        al = (C.c_uint32 * 4)()
+        self._rc(lib().arc_chunk_info(self._h, idx, C.byref(nn), C.byref(nb), C.byref(sv), al), "arc_chunk_info")
+        return {"n_nodes": nn.value, "n_bricks": nb.value, "solid": sv.value, "alloc": tuple(al)}
+
+    def voxel(self, x: int, y: int, z: int) -> int:
+        return int(lib().arc_world_host_voxel(self._h, x, y, z))
+
+    def sync_chunk(self, idx: int) -> None:
+        self._rc(lib().arc_world_host_sync_chunk(self._h, idx), "sync_chunk")
+
+    def sync_chunk_partial(self, idx: int, rep: EditReport) -> None:
+        self._rc(lib().arc_world_host_sync_chunk_partial(self._h, idx, C.byref(rep)), "sync_chunk_partial")
+
+    def set_voxels(self, voxels) -> int:
+        """WorldMap::set_voxels_global; voxels = (x, y, z, mat) rows in global voxel coordinates."""
+        v = np.ascontiguousarray(np.asarray(voxels, dtype=np.int32).reshape(-1, 4))
+        k = C.c_uint32()
+        self._rc(lib().arc_world_host_set_voxels(self._h, _ptr(v) if v.size else None, v.shape[0], C.byref(k)),
+                 "set_voxels")
+        return k.value
+
+    def clear_voxels(self, voxels) -> int:
+        v = np.ascontiguousarray(np.asarray(voxels, dtype=np.int32).reshape(-1, 3))
+        k = C.c_uint32()
+        self._rc(lib().arc_world_host_clear_voxels(self._h, _ptr(v) if v.size else None, v.shape[0], C.byref(k)),
+                 "clear_voxels")
+        return k.value
+
+    def carve_crater(self, center_m, radius_m: float, scorch_rim: bool = False) -> int:
+        c = (C.c_float * 3)(*center_m)
+        k = C.c_uint32()
+        self._rc(lib().arc_world_host_carve_crater(self._h, c, radius_m, int(scorch_rim), C.byref(k)), "carve_crater")
+        return k.value
+
+    def pending_patches(self):
+        """Queued GpuPatch views (ctypes array, valid until the next edit / clear_patches)."""
+        p, n = C.POINTER(GpuPatch)(), C.c_uint64()
+        self._rc(lib().arc_world_host_pending_patches(self._h, C.byref(p), C.byref(n)), "pending_patches")
+        return p, n.value
+
+    def patch_list(self):
+        """Copies of the queued patches as dicts (nodes / bricks as numpy arrays)."""
+        p, n = self.pending_patches()
+        out = []
+        for i in range(n):
+            q = p[i]
+            nodes = (np.frombuffer((C.c_char * (q.n_nodes * 16)).from_address(q.nodes), NODE_DTYPE).copy()
+                     if q.n_nodes else np.zeros(0, NODE_DTYPE))
+            bricks = (np.frombuffer((C.c_char * (q.n_bricks * 64)).from_address(q.bricks), BRICK_DTYPE).copy()
+                      if q.n_bricks else np.zeros(0, BRICK_DTYPE))
+            out.append({"chunk_flat_idx": q.chunk_flat_idx, "new_root_node_idx": q.new_root_node_idx,
+                        "node_offset": q.node_offset, "nodes": nodes, "brick_offset": q.brick_offset,
+                        "bricks": bricks})
+        return out
+
+    def apply_patches_to(self, roots: np.ndarray, nodes: np.ndarray, bricks: np.ndarray) -> int:
+        """Apply the queued patches to host copies of the pools (arc_world_patch_host, in place)."""
+        p, n = self.pending_patches()
+        if n:
+            self._rc(lib().arc_world_patch_host(p, n, len(roots), _ptr(roots), _ptr(nodes), len(nodes), _ptr(bricks),
+                                                len(bricks)), "arc_world_patch_host")
+        return n
+
+    def clear_patches(self) -> None:
+        self._rc(lib().arc_world_host_clear_patches(self._h), "clear_patches")
+
     def pools(self):
         """Copies of (grid, chunk_roots, node_pool, brick_pool) after pack()."""
         g = (C.c_uint32 * 3)()
@@ -397,6 +534,24 @@ class Context:
 
     def upload_world(self, world: HostWorld, **kw):
         self.upload(*world.pools(), **kw)
+
+    def read_pools(self, n_roots: int, n_nodes: int, n_bricks: int):
+        """Device pools read back: (chunk_roots, nodes[:n_nodes], bricks[:n_bricks])."""
+        roots = np.zeros(n_roots, np.uint32)
+        nodes = np.zeros(n_nodes, NODE_DTYPE)
+        bricks = np.zeros(n_bricks, BRICK_DTYPE)
+        self._check(lib().arc_world_read_pools(self._h, _ptr(roots), _ptr(nodes) if n_nodes else None, 0, n_nodes,
+                                               _ptr(bricks) if n_bricks else None, 0, n_bricks), "read_pools")
+        return roots, nodes, bricks
+
+    def patch_world(self, world: "HostWorld", clear: bool = True) -> int:
+        """Uploads the world's queued patches (Renderer::apply_world_patches) and drains the queue."""
+        p, n = world.pending_patches()
+        if n:
+            self._check(lib().arc_world_patch(self._h, p, n), "arc_world_patch")
+        if clear:
+            world.clear_patches()
+        return n
 
     def patch(self, patches) -> None:
         arr = (GpuPatch * max(len(patches), 1))()

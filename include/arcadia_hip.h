@@ -157,9 +157,19 @@ int arc_world_upload(arc_ctx* ctx, const uint32_t grid[3], const uint32_t* chunk
  * node/brick slices are written at their offsets, then chunk_roots[chunk_flat_idx] is set.
  * Slices beyond the reserved capacity grow the pools (device copy), as the reference does. */
 int arc_world_patch(arc_ctx* ctx, const arc_gpu_patch* patches, uint64_t n);
+/* The same patch application on host pools (roots[n_chunks], nodes[node_cap], bricks[brick_cap]),
+ * with the same validation and the same last-writer-wins staging plan as arc_world_patch; no
+ * reference counterpart (the reference's host mirror is WorldMap::gpu_nodes). ARC_ERR_CAPACITY if
+ * a patch writes past the given capacities. */
+int arc_world_patch_host(const arc_gpu_patch* patches, uint64_t n, uint32_t n_chunks, uint32_t* roots,
+                         arc_node64* nodes, uint64_t node_cap, arc_brick64* bricks, uint64_t brick_cap);
 /* Device addresses of the pools (for callers that build their own launches). */
 int arc_world_device_pools(const arc_ctx* ctx, const uint32_t** roots, const arc_node64** nodes,
                            const arc_brick64** bricks);
+/* Read back the device pools (chunk_roots whole, node / brick index ranges) into host arrays
+ * (no reference counterpart: the reference never reads its pools back). Blocking. */
+int arc_world_read_pools(arc_ctx* ctx, uint32_t* roots, arc_node64* nodes, uint64_t node_first, uint64_t n_nodes,
+                         arc_brick64* bricks, uint64_t brick_first, uint64_t n_bricks);
 
 /* ---- traversal ------------------------------------------------------------------------- */
 /* Host-pointer convenience: copies rays in, traces, copies hits out, synchronises. */

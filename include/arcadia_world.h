@@ -106,6 +106,62 @@ int arc_gen_primary_rays(const arc_camera* cam, uint32_t width, uint32_t height,
 int64_t arc_gen_bounce_rays(const arc_ray* primary_rays, const arc_hit* primary_hits, uint64_t n,
                             uint64_t seed, float max_dist_m, arc_ray* rays, uint32_t* src_index);
 
+/* ---- in-place edits and GPU patches ----------------------------------------------------- */
+/* ChunkEditReport (crates/arcadia-core/src/chunk_tree/edit/set.rs:17-80): dirty index ranges of a
+ * chunk's local node/brick arrays; min = UINT64_MAX and max = 0 when nothing is dirty. */
+typedef struct arc_edit_report {
+  uint64_t min_dirty_node, max_dirty_node;
+  uint64_t min_dirty_brick, max_dirty_brick;
+  uint32_t added_voxels;          /* air -> solid transitions */
+  uint32_t had_structural_growth; /* nodes or bricks were appended */
+  uint32_t reserved[2];
+} arc_edit_report;
+
+/* Chunk-local edits (no pool sync). Coordinates are chunk-local voxels in 0..256; entries outside
+ * are skipped, as the reference does.
+ *   arc_chunk_set_voxels   ChunkTree64::set_voxels_local_tracked (set.rs:84-138); xyzm = 4 u32
+ *                          per voxel (x, y, z, material).
+ *   arc_chunk_clear_voxels ChunkTree64::clear_voxels_local_tracked (clear.rs:24-53); xyz = 3 u32.
+ *   arc_chunk_carve_sphere ChunkTree64::carve_sphere_tracked (carve.rs:39-177); center and radius
+ *                          in chunk-local voxels; scorch_rim as the reference (mats 23/24 rim).
+ *   arc_chunk_compact      ChunkTree64::compact (set.rs:156-209).
+ *   arc_chunk_info         node/brick counts, solid voxel count, pool allocation
+ *                          {node_base, node_cap, brick_base, brick_cap} (world_map.rs chunk_allocs). */
+int arc_chunk_set_voxels(arc_world_host* w, uint32_t chunk_flat_idx, const uint32_t* xyzm, uint64_t n,
+                         arc_edit_report* report);
+int arc_chunk_clear_voxels(arc_world_host* w, uint32_t chunk_flat_idx, const uint32_t* xyz, uint64_t n,
+                           uint32_t* removed, arc_edit_report* report);
+int arc_chunk_carve_sphere(arc_world_host* w, uint32_t chunk_flat_idx, const float center[3], float radius,
+                           int scorch_rim, uint32_t* destroyed, arc_edit_report* report);
+int arc_chunk_compact(arc_world_host* w, uint32_t chunk_flat_idx);
+int arc_chunk_info(const arc_world_host* w, uint32_t chunk_flat_idx, uint64_t* n_nodes, uint64_t* n_bricks,
+                   uint64_t* solid_voxels, uint32_t alloc[4]);
+/* Material at global voxel coordinates (WorldMap::get_voxel, world_map.rs:387-395); 0 outside. */
+uint32_t arc_world_host_voxel(const arc_world_host* w, int32_t x, int32_t y, int32_t z);
+
+/* Pool sync after edits (the world must be packed): WorldMap::sync_dirty_chunk
+ * (world_map.rs:208-315) and sync_dirty_chunk_partial (:319-383). They update the host pools
+ * returned by arc_world_host_pools (whose arrays may move) and queue GPU patches. */
+int arc_world_host_sync_chunk(arc_world_host* w, uint32_t chunk_flat_idx);
+int arc_world_host_sync_chunk_partial(arc_world_host* w, uint32_t chunk_flat_idx, const arc_edit_report* report);
+
+/* World-level edits in global coordinates, each followed by the partial sync of every touched
+ * chunk when the world is packed:
+ *   arc_world_host_set_voxels    WorldMap::set_voxels_global (world_map.rs:586-621); xyzm = 4 i32.
+ *   arc_world_host_clear_voxels  WorldMap::clear_voxels_global (:549-583); chunks in ascending index.
+ *   arc_world_host_carve_crater  the tree part of WorldMap::carve_crater
+ *                                (crates/arcadia-world/src/destruction.rs:47-155): center and radius in
+ *                                meters; no debris particles, no island detection. */
+int arc_world_host_set_voxels(arc_world_host* w, const int32_t* xyzm, uint64_t n, uint32_t* added);
+int arc_world_host_clear_voxels(arc_world_host* w, const int32_t* xyz, uint64_t n, uint32_t* removed);
+int arc_world_host_carve_crater(arc_world_host* w, const float center_m[3], float radius_m, int scorch_rim,
+                                uint32_t* destroyed);
+
+/* Queued GPU patches (WorldMap::pending_patches), in order, as views into the world valid until its
+ * next edit, sync, pack or arc_world_host_clear_patches. Feed them to arc_world_patch. */
+int arc_world_host_pending_patches(arc_world_host* w, const arc_gpu_patch** patches, uint64_t* n);
+int arc_world_host_clear_patches(arc_world_host* w);
+
 #ifdef __cplusplus
 }
 #endif
