@@ -16,17 +16,20 @@ from .native import (  # noqa: F401
     RAY_DTYPE,
     RESULT_DTYPE,
     ArcError,
+    Batch,
     Context,
     GbufferParams,
     HostWorld,
     TraceParams,
     bounce_rays,
+    bounce_rays_dense,
     camera_look,
     gen_params,
     lib,
     lod_payload,
     palette_material,
     primary_rays,
+    view_rays,
 )
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -41,6 +41,8 @@ struct arc_ctx {
   arc_ray* d_gb_rays = nullptr;
   arc_trace_result* d_gb_res = nullptr;
   uint64_t gb_n = 0;
+  // bumped whenever the device pool addresses change (upload, pool growth): captured graphs bake them
+  uint64_t world_gen = 0;
   // arc_world_patch staging (pinned host + device copy of the same size)
   void* h_stage = nullptr;
   void* d_stage = nullptr;
@@ -282,6 +284,7 @@ int arc_world_upload(arc_ctx* c, const uint32_t grid[3], const uint32_t* roots, 
   c->n_bricks = n_bricks;
   c->brick_cap = brick_cap;
   c->has_world = true;
+  c->world_gen++;
   return ARC_OK;
 }
 
@@ -337,6 +340,7 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
     (void)hipFree(c->d_nodes);
     c->d_nodes = nn;
     c->node_cap = cap;
+    c->world_gen++;
   }
   if (plan.need_bricks > c->brick_cap) {
     const uint64_t cap =
@@ -349,6 +353,7 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
     (void)hipFree(c->d_bricks);
     c->d_bricks = nb;
     c->brick_cap = cap;
+    c->world_gen++;
   }
   const size_t bytes = plan.total_units * 16;
   if (bytes > c->stage_bytes) {
@@ -550,6 +555,248 @@ int arc_trace(arc_ctx* c, const arc_trace_params* params, const arc_ray* rays, u
   HIP_TRY(c, hipDeviceSynchronize());
   HIP_TRY(c, hipMemcpy(hits_out, c->d_hits, n * sizeof(arc_hit), hipMemcpyDeviceToHost));
   return ARC_OK;
+}
+
+// ---- batched views: one persistent path launch ------------------------------------------------
+}  // extern "C"
+
+struct arc_batch {
+  arc_ctx* ctx = nullptr;
+  arcdev::ViewDev* d_views = nullptr;
+  uint32_t* d_first = nullptr;
+  std::vector<uint64_t> first;  // host copy, n_views + 1
+  uint32_t n_views = 0;
+  uint64_t n_items = 0;
+  uint32_t n_bounces = 0;
+  float bounce_tmax = 0.0f;
+  uint64_t seed = 0;
+  arc_trace_params primary{}, bounce{};
+  arc_launch_config cfg{};
+  // captured graph of one launch
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap_stream = nullptr;
+  uint64_t graph_gen = 0;
+};
+
+namespace {
+
+int batch_launch(arc_batch* b, arc_hit* d_hits, hipStream_t s) {
+  arc_ctx* c = b->ctx;
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "trace before arc_world_upload");
+  arcdev::PathLaunchArgs a{};
+  std::memcpy(a.grid, c->grid, sizeof a.grid);
+  a.roots = c->d_roots;
+  a.nodes = c->d_nodes;
+  a.bricks = c->d_bricks;
+  a.primary = b->primary;
+  a.bounce = b->bounce;
+  a.views = b->d_views;
+  a.view_first = b->d_first;
+  a.n_views = b->n_views;
+  a.n_items = (uint32_t)b->n_items;
+  a.n_bounces = b->n_bounces;
+  a.bounce_max_dist_m = b->bounce_tmax;
+  a.seed = b->seed;
+  a.out = d_hits;
+  a.queue = c->d_queue + 8;  // the path kernel's own counter pair
+  uint32_t bpc = b->cfg.blocks_per_cu;
+  if (bpc == 0) {
+    hipError_t oe = arcdev::path_occupancy(b->cfg.block_threads, &bpc);
+    if (oe != hipSuccess) return fail(c, ARC_ERR_HIP, "occupancy query: %s", hipGetErrorString(oe));
+  }
+  a.block_threads = b->cfg.block_threads;
+  a.blocks = std::max<uint32_t>(1u, c->n_cu * bpc);
+  a.refill = std::max<uint32_t>(1u, std::min<uint32_t>(b->cfg.refill_lanes, 64u));
+  a.batch = std::max<uint32_t>(1u, b->cfg.batch_rays);
+  hipError_t e = arcdev::launch_path(a, s);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "path kernel launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int arc_batch_create(arc_ctx* c, const arc_view* views, uint32_t n_views, const arc_trace_params* primary,
+                     const arc_trace_params* bounce, uint32_t n_bounces, float bounce_max_dist_m, uint64_t seed,
+                     arc_batch** out) {
+  if (!c || !out) return ARC_ERR_INVALID;
+  *out = nullptr;
+  if (!views || n_views == 0 || n_views > 65536) return fail(c, ARC_ERR_INVALID, "1..65536 views");
+  if (!valid_params(primary) || !valid_params(bounce) || primary->semantics != ARC_SEM_WORLD ||
+      bounce->semantics != ARC_SEM_WORLD)
+    return fail(c, ARC_ERR_INVALID, "batches trace world semantics");
+  if (n_bounces > 3) return fail(c, ARC_ERR_INVALID, "at most 3 bounces");
+  std::vector<arcdev::ViewDev> hv(n_views);
+  std::vector<uint32_t> first(n_views + 1);
+  uint64_t items = 0;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    const arc_view& iv = views[v];
+    if (iv.width == 0 || iv.height == 0 || iv.width > 16384 || iv.height > 16384)
+      return fail(c, ARC_ERR_INVALID, "view %u: size must be 1..16384 per axis", v);
+    hv[v].cam = iv.cam;
+    hv[v].width = iv.width;
+    hv[v].height = iv.height;
+    hv[v].tiles_x = (iv.width + 7) / 8;
+    hv[v].max_dist_m = iv.max_dist_m;
+    first[v] = (uint32_t)items;
+    items += (uint64_t)hv[v].tiles_x * ((iv.height + 7) / 8) * 64u;
+    if (items * (1u + n_bounces) > 0x7FFFFFFFull) return fail(c, ARC_ERR_CAPACITY, "batch too large");
+  }
+  first[n_views] = (uint32_t)items;
+  arc_batch* b = new (std::nothrow) arc_batch();
+  if (!b) return ARC_ERR_CAPACITY;
+  b->ctx = c;
+  b->n_views = n_views;
+  b->n_items = items;
+  b->n_bounces = n_bounces;
+  b->bounce_tmax = bounce_max_dist_m;
+  b->seed = seed;
+  b->primary = *primary;
+  b->bounce = *bounce;
+  b->cfg = default_config();
+  b->first.assign(first.begin(), first.end());
+  hipError_t e = hipSetDevice(c->device);
+  if (e == hipSuccess) e = hipMalloc(&b->d_views, n_views * sizeof(arcdev::ViewDev));
+  if (e == hipSuccess) e = hipMalloc(&b->d_first, (n_views + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(b->d_views, hv.data(), n_views * sizeof(arcdev::ViewDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_first, first.data(), (n_views + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(b->d_views);
+    (void)hipFree(b->d_first);
+    delete b;
+    return fail(c, ARC_ERR_HIP, "batch upload: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return ARC_OK;
+}
+
+uint64_t arc_batch_slots(const arc_batch* b) { return b ? b->n_items : 0; }
+
+uint64_t arc_batch_view_first(const arc_batch* b, uint32_t v) {
+  return (b && v <= b->n_views) ? b->first[v] : 0;
+}
+
+int arc_batch_set_launch_config(arc_batch* b, const arc_launch_config* cfg) {
+  if (!b || !cfg) return ARC_ERR_INVALID;
+  arc_launch_config n;
+  int rc = normalize_config(b->ctx, cfg, &n);
+  if (rc != ARC_OK) return rc;
+  b->cfg = n;
+  return ARC_OK;
+}
+
+int arc_batch_get_launch_config(const arc_batch* b, arc_launch_config* cfg) {
+  if (!b || !cfg) return ARC_ERR_INVALID;
+  *cfg = b->cfg;
+  return ARC_OK;
+}
+
+int arc_batch_trace(arc_batch* b, arc_hit* d_hits, void* stream) {
+  if (!b) return ARC_ERR_INVALID;
+  if (!d_hits) return fail(b->ctx, ARC_ERR_INVALID, "null hit buffer");
+  HIP_TRY(b->ctx, hipSetDevice(b->ctx->device));
+  return batch_launch(b, d_hits, (hipStream_t)stream);
+}
+
+int arc_batch_capture(arc_batch* b, arc_hit* d_hits) {
+  if (!b) return ARC_ERR_INVALID;
+  arc_ctx* c = b->ctx;
+  if (!d_hits) return fail(c, ARC_ERR_INVALID, "null hit buffer");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  if (!b->cap_stream) HIP_TRY(c, hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+  HIP_TRY(c, hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+  const int rc = batch_launch(b, d_hits, b->cap_stream);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(b->cap_stream, &g);
+  if (rc != ARC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess) return fail(c, ARC_ERR_HIP, "graph capture: %s", hipGetErrorString(ee));
+  b->graph = g;
+  HIP_TRY(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_gen = c->world_gen;
+  return ARC_OK;
+}
+
+int arc_batch_replay(arc_batch* b, void* stream) {
+  if (!b) return ARC_ERR_INVALID;
+  arc_ctx* c = b->ctx;
+  if (!b->exec) return fail(c, ARC_ERR_INVALID, "arc_batch_replay before arc_batch_capture");
+  if (b->graph_gen != c->world_gen)
+    return fail(c, ARC_ERR_INVALID, "world pools moved (upload or growth) since the capture: capture again");
+  HIP_TRY(c, hipGraphLaunch(b->exec, (hipStream_t)stream));
+  return ARC_OK;
+}
+
+int arc_batch_autotune(arc_batch* b, arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
+                       float* best_ms) {
+  if (!b) return ARC_ERR_INVALID;
+  arc_ctx* c = b->ctx;
+  if (!d_hits) return fail(c, ARC_ERR_INVALID, "null hit buffer");
+  if (reps == 0) reps = 3;
+  if (reps > 64) reps = 64;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  static const uint32_t kThreads[] = {128, 256};
+  static const uint32_t kBlocks[] = {0, 3, 4};
+  static const uint32_t kRefill[] = {8, 16, 24, 32, 48};
+  static const uint32_t kBatch[] = {32, 64, 128};
+  const arc_launch_config saved = b->cfg;
+  arc_launch_config win = saved;
+  float win_ms = 1e30f;
+  int rc = ARC_OK;
+  std::vector<float> t(reps);
+  for (uint32_t bt : kThreads)
+    for (uint32_t bpc : kBlocks)
+      for (uint32_t rf : kRefill)
+        for (uint32_t bat : kBatch) {
+          if (rc != ARC_OK) break;
+          b->cfg = arc_launch_config{bpc, bt, rf, bat};
+          rc = batch_launch(b, d_hits, s);
+          for (uint32_t r = 0; r < reps && rc == ARC_OK; ++r) {
+            if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK) rc = batch_launch(b, d_hits, s);
+            if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
+          }
+          if (rc != ARC_OK) break;
+          std::vector<float> sorted = t;
+          std::sort(sorted.begin(), sorted.end());
+          if (sorted[reps / 2] < win_ms) {
+            win_ms = sorted[reps / 2];
+            win = b->cfg;
+          }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->cfg = rc == ARC_OK ? win : saved;
+  if (rc != ARC_OK) return rc;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return ARC_OK;
+}
+
+void arc_batch_destroy(arc_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  (void)hipFree(b->d_views);
+  (void)hipFree(b->d_first);
+  delete b;
 }
 
 }  // extern "C"

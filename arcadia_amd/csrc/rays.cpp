@@ -4,40 +4,10 @@
 #include <cmath>
 #include <cstring>
 
+#include "raymath.h"
 #include "world_internal.h"
 
-namespace {
-
-inline uint64_t splitmix64(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-// normalize(v) = v / sqrt(dot(v, v)), dot summed x, y, z left to right.
-inline void normalize3(const float v[3], float out[3]) {
-  float xx = v[0] * v[0];
-  float yy = v[1] * v[1];
-  float zz = v[2] * v[2];
-  float d = xx + yy;
-  d = d + zz;
-  float len = std::sqrt(d);
-  out[0] = v[0] / len;
-  out[1] = v[1] / len;
-  out[2] = v[2] / len;
-}
-
-inline void cross3(const float a[3], const float b[3], float o[3]) {
-  float x0 = a[1] * b[2], x1 = a[2] * b[1];
-  float y0 = a[2] * b[0], y1 = a[0] * b[2];
-  float z0 = a[0] * b[1], z1 = a[1] * b[0];
-  o[0] = x0 - x1;
-  o[1] = y0 - y1;
-  o[2] = z0 - z1;
-}
-
-}  // namespace
+using arcmath::normalize3;
 
 extern "C" {
 
@@ -78,26 +48,8 @@ int arc_gen_primary_rays(const arc_camera* cam, uint32_t width, uint32_t height,
       for (uint32_t j = 0; j < 64; ++j) {
         uint32_t px = tx * 8 + (j & 7), py = ty * 8 + (j >> 3);
         if (px >= width || py >= height) continue;
-        // primary_gbuffer.wgsl:103-115
-        float ux = (float)px + 0.5f;
-        ux = ux + cam->jitter[0];
-        float uy = (float)py + 0.5f;
-        uy = uy + cam->jitter[1];
-        float uvx = ux / fw, uvy = uy / fh;
-        float ndx = uvx * 2.0f - 1.0f;
-        float ndy = 1.0f - uvy * 2.0f;
-        float sx = ndx * cam->aspect;
-        sx = sx * cam->tan_half_fov;
-        float sy = ndy * cam->tan_half_fov;
-        float v[3];
-        for (int a = 0; a < 3; ++a) {
-          float rr = cam->right[a] * sx;
-          float uu = cam->up[a] * sy;
-          float s = cam->forward[a] + rr;
-          v[a] = s + uu;
-        }
         float d[3];
-        normalize3(v, d);
+        arcmath::primary_dir(*cam, px, py, width, height, d);
         arc_ray& r = rays[k];
         r.ox = cam->eye[0];
         r.oy = cam->eye[1];
@@ -121,53 +73,57 @@ int64_t arc_gen_bounce_rays(const arc_ray* prays, const arc_hit* hits, uint64_t 
     const arc_hit& h = hits[i];
     if (!(h.flags & ARC_HIT_FLAG_HIT)) continue;
     const arc_ray& pr = prays[i];
-    const float o[3] = {pr.ox, pr.oy, pr.oz}, dir[3] = {pr.dx, pr.dy, pr.dz};
-    float nrm[3] = {0.0f, 0.0f, 0.0f};
-    if (h.face < 6) nrm[h.face >> 1] = (h.face & 1) ? -1.0f : 1.0f;
-    float hp[3], org[3];
-    for (int a = 0; a < 3; ++a) {
-      float vo = o[a] * 16.0f;
-      float dt = dir[a] * h.t_voxels;
-      float pv = vo + dt;
-      hp[a] = pv * 0.0625f;
-      float off = nrm[a] * 0.10f;
-      org[a] = hp[a] + off;
-    }
-    // lighting_restir_gi.wgsl:195-197; types.wgsl:483-497, 531-545
-    const float lifted[3] = {nrm[0], nrm[1] + 0.22f, nrm[2]};
-    float sn[3];
-    normalize3(lifted, sn);
-    const float upv[3] = {std::fabs(sn[2]) < 0.999f ? 0.0f : 1.0f, 0.0f, std::fabs(sn[2]) < 0.999f ? 1.0f : 0.0f};
-    float c[3], t[3], b[3];
-    cross3(upv, sn, c);
-    normalize3(c, t);
-    cross3(sn, t, b);
-    uint64_t hsh = splitmix64(seed ^ splitmix64(i + 0x51ED270Bull));
-    float u1 = (float)(uint32_t)(hsh & 0xFFFFFFu) * (1.0f / 16777216.0f);
-    float u2 = (float)(uint32_t)((hsh >> 24) & 0xFFFFFFu) * (1.0f / 16777216.0f);
-    float r = std::sqrt(u1);
-    float phi = 6.2831853f * u2;
-    float z = std::sqrt(std::fmax(0.0f, 1.0f - u1));
-    float rc = r * std::cos(phi), rs = r * std::sin(phi);
-    float d[3];
-    for (int a = 0; a < 3; ++a) {
-      float p0 = t[a] * rc, p1 = b[a] * rs, p2 = sn[a] * z;
-      float s = p0 + p1;
-      d[a] = s + p2;
-    }
-    arc_ray& out = rays[k];
-    out.ox = org[0];
-    out.oy = org[1];
-    out.oz = org[2];
-    out.t_start = 0.0f;
-    out.dx = d[0];
-    out.dy = d[1];
-    out.dz = d[2];
-    out.t_max = max_dist_m;
+    const float ovox[3] = {pr.ox * 16.0f, pr.oy * 16.0f, pr.oz * 16.0f};
+    const float dir[3] = {pr.dx, pr.dy, pr.dz};
+    arcmath::bounce_ray(ovox, dir, h.t_voxels, h.face, seed, i, max_dist_m, &rays[k]);
     if (src_index) src_index[k] = (uint32_t)i;
     ++k;
   }
   return k;
+}
+
+int64_t arc_gen_view_rays(const arc_camera* cam, uint32_t width, uint32_t height, float max_dist_m, arc_ray* rays) {
+  if (!cam || !rays || width == 0 || height == 0 || width > 65535 || height > 65535) return ARC_ERR_INVALID;
+  const uint32_t tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
+  uint64_t k = 0;
+  for (uint32_t ty = 0; ty < tiles_y; ++ty)
+    for (uint32_t tx = 0; tx < tiles_x; ++tx)
+      for (uint32_t j = 0; j < 64; ++j, ++k) {
+        const uint32_t px = tx * 8 + (j & 7), py = ty * 8 + (j >> 3);
+        if (px >= width || py >= height) {
+          rays[k] = arcmath::empty_ray();
+          continue;
+        }
+        float d[3];
+        arcmath::primary_dir(*cam, px, py, width, height, d);
+        arc_ray& r = rays[k];
+        r.ox = cam->eye[0];
+        r.oy = cam->eye[1];
+        r.oz = cam->eye[2];
+        r.t_start = 0.0f;
+        r.dx = d[0];
+        r.dy = d[1];
+        r.dz = d[2];
+        r.t_max = max_dist_m;
+      }
+  return (int64_t)k;
+}
+
+int arc_gen_bounce_rays_dense(const arc_ray* prays, const arc_hit* hits, uint64_t n, uint64_t index_base,
+                              uint64_t seed, float max_dist_m, arc_ray* rays) {
+  if ((!prays || !hits || !rays) && n) return ARC_ERR_INVALID;
+  for (uint64_t i = 0; i < n; ++i) {
+    const arc_hit& h = hits[i];
+    if (!(h.flags & ARC_HIT_FLAG_HIT)) {
+      rays[i] = arcmath::empty_ray();
+      continue;
+    }
+    const arc_ray& pr = prays[i];
+    const float ovox[3] = {pr.ox * 16.0f, pr.oy * 16.0f, pr.oz * 16.0f};
+    const float dir[3] = {pr.dx, pr.dy, pr.dz};
+    arcmath::bounce_ray(ovox, dir, h.t_voxels, h.face, seed, index_base + i, max_dist_m, &rays[i]);
+  }
+  return ARC_OK;
 }
 
 }  // extern "C"

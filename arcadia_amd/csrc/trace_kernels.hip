@@ -28,6 +28,7 @@
 // node: graph-safe).
 #include <hip/hip_runtime.h>
 
+#include "raymath.h"
 #include "trace_device.h"
 #include "trace_launch.h"
 
@@ -42,6 +43,7 @@ enum : uint32_t {
   F_LIVE = 6,    // 1 bit: ray in flight
   F_HIT = 7,     // 1 bit: finished with a hit
   F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
+  F_DEPTH = 9,   // 2 bits: path kernel: 0 = primary ray, d = d-th bounce
 };
 
 __device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
@@ -542,6 +544,177 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Path kernel: batched views with in-lane bounces
+// ---------------------------------------------------------------------------------------
+// One persistent launch traces every pixel slot of a batch of views and, for each primary ray
+// that hits, up to `n_bounces` diffuse bounces generated in the lane from the finished hit
+// (arcmath::bounce_ray, bit-identical to arc_gen_bounce_rays_dense). A queue item is one pixel
+// slot: views are laid out one after another in padded 8x8-tile order (view_first[v] = first slot
+// of view v). Records: out[d * n_items + slot] = the compact hit of depth d (0 = primary); slots
+// without a ray at depth d (padding pixels, bounces of misses) get the record of the empty ray
+// (t_voxels -16, no hit, 0 steps), the same the traversal writes for a ray with t_max = -1.
+
+__device__ __forceinline__ DevParams sel_params(bool b, const DevParams& pb, const DevParams& p0) {
+  DevParams p;
+  p.lod_bias = b ? pb.lod_bias : p0.lod_bias;
+  p.ignore_water = b ? pb.ignore_water : p0.ignore_water;
+  p.max_steps = b ? pb.max_steps : p0.max_steps;
+  p.lod_step = b ? pb.lod_step : p0.lod_step;
+  p.lod_t = b ? pb.lod_t : p0.lod_t;
+  p.lod_t_far = b ? pb.lod_t_far : p0.lod_t_far;
+  p.ao_max_dist = b ? pb.ao_max_dist : p0.ao_max_dist;
+  p.pad = 0;
+  return p;
+}
+
+__device__ __forceinline__ uint4 empty_record() {
+  return make_uint4(__float_as_uint(-16.0f), 0u, (uint32_t)ARC_FACE_NONE << 24, 0u);
+}
+
+struct PathCtl {
+  const ViewDev* views;
+  const uint32_t* view_first;  // n_views + 1 entries, view_first[n_views] = n_items
+  uint32_t n_views, n_items, n_bounces;
+  float bounce_tmax;
+  uint64_t seed;
+};
+
+// Starts slot `item`: its primary ray, or the empty records of a padding slot.
+__device__ __forceinline__ void path_start(const DevWorld& w, const DevParams& p0, const PathCtl& pc, uint32_t item,
+                                           uint4* out, Lane& L, float& tmax, uint4* C) {
+  uint32_t lo = 0u, hi = pc.n_views;  // last view with view_first <= item
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pc.view_first[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  const ViewDev v = pc.views[lo];
+  const uint32_t local = item - pc.view_first[lo];
+  const uint32_t tile = local >> 6, j = local & 63u;
+  const uint32_t px = (tile % v.tiles_x) * 8u + (j & 7u), py = (tile / v.tiles_x) * 8u + (j >> 3);
+  L.idx = item;
+  if (px >= v.width || py >= v.height) {
+    for (uint32_t d = 0; d <= pc.n_bounces; ++d) out[(size_t)d * pc.n_items + item] = empty_record();
+    L.st = 0u;
+    return;
+  }
+  float dir[3];
+  arcmath::primary_dir(v.cam, px, py, v.width, v.height, dir);
+  arc_ray r;
+  r.ox = v.cam.eye[0];
+  r.oy = v.cam.eye[1];
+  r.oz = v.cam.eye[2];
+  r.t_start = 0.0f;
+  r.dx = dir[0];
+  r.dy = dir[1];
+  r.dz = dir[2];
+  r.t_max = v.max_dist_m;
+  tmax = v.max_dist_m;
+  ray_init<ARC_SEM_WORLD>(w, p0, r, L, C);
+}
+
+// Writes the finished ray's record, then starts its bounce or closes the slot.
+__device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& pb, const PathCtl& pc, uint4* out,
+                                            Lane& L, float& tmax, uint4* C) {
+  const uint32_t depth = fget(L.st, F_DEPTH, 2);
+  const uint32_t steps = L.step > 0xFFFFu ? 0xFFFFu : L.step;
+  const bool hit = fget(L.st, F_HIT, 1) != 0u;
+  uint4 v;
+  uint32_t face = ARC_FACE_NONE;
+  float exact_t = 0.0f;
+  if (hit) {
+    const HitInfo h = resolve_hit<ARC_SEM_WORLD>(L, L.dx, L.dy, L.dz);
+    face = h.nrm_axis > 0.0f ? 2u * h.axis : (h.nrm_axis < 0.0f ? 2u * h.axis + 1u : 6u + h.axis);
+    exact_t = h.exact_t;
+    v.x = __float_as_uint(h.exact_t);
+    v.y = (L.vx & 0xFFFFu) | (L.vy << 16);
+    v.z = (L.vz & 0xFFFFu) | ((L.mat & 0xFFu) << 16) | (face << 24);
+    v.w = steps | ((ARC_HIT_FLAG_HIT | (fget(L.st, F_SHIFT, 3) > 0u ? ARC_HIT_FLAG_LOD : 0u)) << 16);
+  } else {
+    v.x = __float_as_uint(tmax * 16.0f);
+    v.y = 0u;
+    v.z = (uint32_t)ARC_FACE_NONE << 24;
+    v.w = steps;
+  }
+  const uint32_t item = L.idx;
+  out[(size_t)depth * pc.n_items + item] = v;
+  if (hit && depth < pc.n_bounces) {
+    const float ov[3] = {L.ox, L.oy, L.oz}, dr[3] = {L.dx, L.dy, L.dz};
+    arc_ray r;
+    arcmath::bounce_ray(ov, dr, exact_t, face, pc.seed, (uint64_t)depth * pc.n_items + item, pc.bounce_tmax, &r);
+    ray_init<ARC_SEM_WORLD>(w, pb, r, L, C);
+    L.st = fput(L.st, F_DEPTH, 2, depth + 1u);
+    tmax = pc.bounce_tmax;
+  } else {
+    for (uint32_t d = depth + 1u; d <= pc.n_bounces; ++d) out[(size_t)d * pc.n_items + item] = empty_record();
+    L.st = fput(L.st, F_HASRAY, 1, 0u);
+  }
+}
+
+// queue[0]: slots claimed, queue[1]: finished workgroups (both zero between launches).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
+    DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
+    uint32_t refill, uint32_t batch) {
+  __shared__ uint4 s_cold[4 * kMaxBlock];
+  uint4* const C = s_cold + threadIdx.x;
+  const uint32_t lane = __lane_id();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+  Lane L;
+  L.st = 0u;
+  L.step = 0u;
+  L.idx = 0u;
+  float tmax = 0.0f;
+  uint32_t wb = 0u, we = 0u;
+  bool drained = false;
+  for (;;) {
+    const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
+    const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
+    const bool pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1)) != 0ull;
+    if (drained ? pending : nidle >= refill) {
+      if (idle_lane && fget(L.st, F_HASRAY, 1)) path_finish(w, pb, pc, out, L, tmax, C);
+      if (!drained) {
+        const bool needs = fget(L.st, F_LIVE, 1) == 0u && fget(L.st, F_HASRAY, 1) == 0u;
+        const uint64_t nm = __ballot(needs);
+        const uint32_t nneed = (uint32_t)__popcll(nm);
+        const uint32_t rank = (uint32_t)__popcll(nm & lt_mask);
+        uint32_t assigned = 0u, my = 0xFFFFFFFFu;
+        while (assigned < nneed) {
+          if (wb >= we) {
+            uint32_t got = 0u;
+            if (lane == 0) got = atomicAdd(queue, batch);
+            got = rfl(__shfl(got, 0));
+            if (got >= pc.n_items) {
+              drained = true;
+              break;
+            }
+            wb = got;
+            we = umin_(got + batch, pc.n_items);
+          }
+          const uint32_t take = umin_(nneed - assigned, we - wb);
+          if (needs && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);
+          wb += take;
+          assigned += take;
+        }
+        if (my != 0xFFFFFFFFu) path_start(w, p0, pc, my, out, L, tmax, C);
+      }
+      continue;
+    }
+    if (drained && nidle == 64u) break;
+    if (!idle_lane) iteration<ARC_SEM_WORLD>(w, sel_params(fget(L.st, F_DEPTH, 2) != 0u, pb, p0), L, C);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t done = atomicAdd(queue + 1, 1u);
+    if (done == gridDim.x - 1u) {
+      atomicExch(queue, 0u);
+      atomicExch(queue + 1, 0u);
+      __threadfence();
+    }
+  }
+}
+
 }  // namespace arcdev
 
 // ---------------------------------------------------------------------------------------
@@ -604,6 +777,36 @@ hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
       hipLaunchKernelGGL((k_trace<ARC_SEM_WORLD, false>), grid, block, 0, stream, w, p, a.rays,
                          reinterpret_cast<arc_hit*>(a.out), n, a.queue, a.refill, a.batch);
   }
+  return hipGetLastError();
+}
+
+hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_path),
+                                                               (int)block_threads, 0);
+  *blocks_per_cu = nb > 0 ? (uint32_t)nb : 1u;
+  return e;
+}
+
+hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream) {
+  DevWorld w;
+  w.gx = a.grid[0];
+  w.gy = a.grid[1];
+  w.gz = a.grid[2];
+  w.pad = 0;
+  w.roots = a.roots;
+  w.nodes = reinterpret_cast<const uint4*>(a.nodes);
+  w.bricks = reinterpret_cast<const uint32_t*>(a.bricks);
+  PathCtl pc;
+  pc.views = a.views;
+  pc.view_first = a.view_first;
+  pc.n_views = a.n_views;
+  pc.n_items = a.n_items;
+  pc.n_bounces = a.n_bounces;
+  pc.bounce_tmax = a.bounce_max_dist_m;
+  pc.seed = a.seed;
+  hipLaunchKernelGGL(k_path, dim3(a.blocks), dim3(a.block_threads), 0, stream, w, make_params(a.primary),
+                     make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue, a.refill, a.batch);
   return hipGetLastError();
 }
 

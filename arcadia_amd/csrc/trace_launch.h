@@ -4,7 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../../include/arcadia_hip.h"
+#include "../../include/arcadia_world.h"
 
 namespace arcdev {
 
@@ -26,6 +26,31 @@ struct LaunchArgs {
 };
 
 hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream);
+
+// One view of a path batch (device copy).
+struct ViewDev {
+  arc_camera cam;
+  uint32_t width, height, tiles_x;
+  float max_dist_m;
+};
+
+struct PathLaunchArgs {
+  uint32_t grid[3];
+  const uint32_t* roots;
+  const arc_node64* nodes;
+  const arc_brick64* bricks;
+  arc_trace_params primary, bounce;
+  const ViewDev* views;
+  const uint32_t* view_first;  // n_views + 1 entries
+  uint32_t n_views, n_items, n_bounces;
+  float bounce_max_dist_m;
+  uint64_t seed;
+  arc_hit* out;      // (1 + n_bounces) * n_items records
+  uint32_t* queue;   // two u32 counters, zero between launches
+  uint32_t blocks, block_threads, refill, batch;
+};
+hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream);
+hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu);
 // Resident workgroups per CU of the traversal kernel at `block_threads`.
 hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_threads, uint32_t* blocks_per_cu);
 

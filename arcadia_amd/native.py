@@ -106,6 +106,11 @@ class Camera(C.Structure):
                 ("tan_half_fov", C.c_float), ("aspect", C.c_float), ("jitter", C.c_float * 2)]
 
 
+class ArcView(C.Structure):
+    _fields_ = [("cam", Camera), ("width", C.c_uint32), ("height", C.c_uint32), ("max_dist_m", C.c_float),
+                ("reserved", C.c_uint32)]
+
+
 class GpuPatch(C.Structure):
     _fields_ = [("chunk_flat_idx", C.c_uint32), ("new_root_node_idx", C.c_uint32), ("node_offset", C.c_uint32),
                 ("n_nodes", C.c_uint32), ("nodes", C.c_void_p), ("brick_offset", C.c_uint32),
@@ -169,6 +174,19 @@ _SIGS = [
                                         C.c_void_p]),
     ("arc_primary_gbuffer", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(GbufferParams), C.c_void_p,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("arc_batch_create", C.c_int, [C.c_void_p, C.POINTER(ArcView), C.c_uint32, C.POINTER(TraceParams),
+                                   C.POINTER(TraceParams), C.c_uint32, C.c_float, C.c_uint64,
+                                   C.POINTER(C.c_void_p)]),
+    ("arc_batch_destroy", None, [C.c_void_p]),
+    ("arc_batch_slots", C.c_uint64, [C.c_void_p]),
+    ("arc_batch_view_first", C.c_uint64, [C.c_void_p, C.c_uint32]),
+    ("arc_batch_trace", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("arc_batch_capture", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("arc_batch_replay", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("arc_batch_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("arc_batch_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("arc_batch_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig),
+                                     C.POINTER(C.c_float)]),
     ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
@@ -211,6 +229,9 @@ _SIGS = [
                                   C.POINTER(Camera)]),
     ("arc_gen_primary_rays", C.c_int, [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_void_p,
                                        C.c_void_p]),
+    ("arc_gen_view_rays", C.c_int64, [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]),
+    ("arc_gen_bounce_rays_dense", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
+                                            C.c_void_p]),
     ("arc_gen_bounce_rays", C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_float, C.c_void_p,
                                         C.c_void_p]),
 ]
@@ -484,6 +505,30 @@ def primary_rays(cam: Camera, width: int, height: int, max_dist_m: float = 250.0
     return rays, pix
 
 
+def view_rays(cam: Camera, width: int, height: int, max_dist_m: float = 250.0) -> np.ndarray:
+    """Padded 8x8-tile-order primary rays of a whole view (empty rays in the padding slots)."""
+    n = ((width + 7) // 8) * ((height + 7) // 8) * 64
+    rays = np.zeros(n, RAY_DTYPE)
+    k = lib().arc_gen_view_rays(C.byref(cam), width, height, max_dist_m, _ptr(rays))
+    if k != n:
+        raise ArcError(int(k), "view_rays")
+    return rays
+
+
+def bounce_rays_dense(prays: np.ndarray, hits: np.ndarray, index_base: int = 0, seed: int = 1,
+                      max_dist_m: float = 26.0) -> np.ndarray:
+    """One bounce per parent ray (empty ray where the parent missed)."""
+    prays = np.ascontiguousarray(prays, RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    out = np.zeros(len(prays), RAY_DTYPE)
+    if len(prays):
+        rc = lib().arc_gen_bounce_rays_dense(_ptr(prays), _ptr(hits), len(prays), index_base, seed, max_dist_m,
+                                             _ptr(out))
+        if rc != ARC_OK:
+            raise ArcError(rc, "bounce_rays_dense")
+    return out
+
+
 def bounce_rays(prays: np.ndarray, hits: np.ndarray, seed: int = 1, max_dist_m: float = 26.0):
     """(rays, src_index): one cosine bounce per hit primary ray."""
     n = len(prays)
@@ -534,6 +579,10 @@ class Context:
 
     def upload_world(self, world: HostWorld, **kw):
         self.upload(*world.pools(), **kw)
+
+    def batch(self, views, primary: TraceParams, bounce: TraceParams, n_bounces: int = 2,
+              bounce_max_dist_m: float = 26.0, seed: int = 1) -> "Batch":
+        return Batch(self, views, primary, bounce, n_bounces, bounce_max_dist_m, seed)
 
     def read_pools(self, n_roots: int, n_nodes: int, n_bricks: int):
         """Device pools read back: (chunk_roots, nodes[:n_nodes], bricks[:n_bricks])."""
@@ -608,3 +657,60 @@ class Context:
         """Primary G-buffer pass into device buffers of 2, 2 and 1 u32 per pixel (asynchronous)."""
         self._check(lib().arc_primary_gbuffer(self._h, C.byref(params), C.byref(g), d_depth_mat, d_norm_motion,
                                               d_albedo_ao, stream or None), "arc_primary_gbuffer")
+
+
+class Batch:
+    """Batched views traced with their bounces in one persistent launch (arc_batch_*)."""
+
+    def __init__(self, ctx: Context, views, primary: TraceParams, bounce: TraceParams, n_bounces: int = 2,
+                 bounce_max_dist_m: float = 26.0, seed: int = 1):
+        """views: iterable of (Camera, width, height, max_dist_m)."""
+        self.ctx = ctx
+        views = list(views)
+        arr = (ArcView * len(views))()
+        for i, (cam, w, h, md) in enumerate(views):
+            arr[i] = ArcView(cam, w, h, md, 0)
+        self.views = views
+        self.n_bounces = n_bounces
+        h = C.c_void_p()
+        ctx._check(lib().arc_batch_create(ctx._h, arr, len(views), C.byref(primary), C.byref(bounce), n_bounces,
+                                          bounce_max_dist_m, seed, C.byref(h)), "arc_batch_create")
+        self._h = h
+        self.slots = int(lib().arc_batch_slots(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().arc_batch_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def view_first(self, v: int) -> int:
+        return int(lib().arc_batch_view_first(self._h, v))
+
+    def records(self) -> int:
+        return (1 + self.n_bounces) * self.slots
+
+    def trace(self, d_hits: int, stream: int = 0) -> None:
+        self.ctx._check(lib().arc_batch_trace(self._h, d_hits, stream or None), "arc_batch_trace")
+
+    def capture(self, d_hits: int) -> None:
+        self.ctx._check(lib().arc_batch_capture(self._h, d_hits), "arc_batch_capture")
+
+    def replay(self, stream: int = 0) -> None:
+        self.ctx._check(lib().arc_batch_replay(self._h, stream or None), "arc_batch_replay")
+
+    def set_launch_config(self, blocks_per_cu=0, block_threads=0, refill_lanes=0, batch_rays=0):
+        cfg = LaunchConfig(blocks_per_cu, block_threads, refill_lanes, batch_rays)
+        self.ctx._check(lib().arc_batch_set_launch_config(self._h, C.byref(cfg)), "arc_batch_set_launch_config")
+
+    def launch_config(self) -> dict:
+        cfg = LaunchConfig()
+        self.ctx._check(lib().arc_batch_get_launch_config(self._h, C.byref(cfg)), "arc_batch_get_launch_config")
+        return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}
+
+    def autotune(self, d_hits: int, stream: int = 0, reps: int = 3):
+        cfg, ms = LaunchConfig(), C.c_float()
+        self.ctx._check(lib().arc_batch_autotune(self._h, d_hits, stream or None, reps, C.byref(cfg), C.byref(ms)),
+                        "arc_batch_autotune")
+        return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}, ms.value

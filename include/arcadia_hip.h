@@ -135,6 +135,18 @@ typedef struct arc_gpu_patch {
   const arc_brick64* bricks;
 } arc_gpu_patch;
 
+/* Pinhole camera (FrameParams eye/forward/right/up, crates/arcadia-render/src/renderer.rs:19-38):
+ * tan of half the vertical fov, aspect = width / height. */
+typedef struct arc_camera {
+  float eye[3];
+  float forward[3];
+  float right[3];
+  float up[3];
+  float tan_half_fov;
+  float aspect;
+  float jitter[2];
+} arc_camera;
+
 typedef struct arc_ctx arc_ctx;
 
 /* ---- context --------------------------------------------------------------------------- */
@@ -226,6 +238,44 @@ int arc_get_launch_config_class(const arc_ctx* ctx, uint32_t lod_class, arc_laun
 int arc_autotune(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n,
                  arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
                  float* best_ms);
+
+/* ---- batched views (one persistent launch) ----------------------------------------------- */
+/* A batch of camera views of uneven sizes traced with their diffuse bounces in ONE persistent
+ * launch (no reference counterpart: the reference records one primary_raymarch dispatch per view
+ * and one GI dispatch per bounce, renderer.rs:667 / lighting_restir_gi.wgsl:195-209). Each pixel
+ * slot's lane generates the primary ray (primary_gbuffer.wgsl:103-115), and after every hit the
+ * next cosine bounce in-lane, bit-identical to arc_gen_view_rays / arc_gen_bounce_rays_dense
+ * (bounce d of slot i hashed with index (d-1) * slots + i). Views are laid out one after another
+ * in padded 8x8-tile order; hits[d * slots + i] is the compact record of depth d (0 = primary);
+ * slots with no ray at that depth (padding pixels, bounces of misses) hold the empty-ray record
+ * (t_voxels -16, no hit, 0 steps). */
+typedef struct arc_view {
+  arc_camera cam;      /* pinhole camera */
+  uint32_t width, height;
+  float max_dist_m;    /* primary ray length (250 m in the reference) */
+  uint32_t reserved;
+} arc_view;
+typedef struct arc_batch arc_batch;
+int arc_batch_create(arc_ctx* ctx, const arc_view* views, uint32_t n_views, const arc_trace_params* primary,
+                     const arc_trace_params* bounce, uint32_t n_bounces, float bounce_max_dist_m, uint64_t seed,
+                     arc_batch** out);
+void arc_batch_destroy(arc_batch* b);
+/* Pixel slots (padded) of the batch; the hit buffer holds (1 + n_bounces) * slots records. */
+uint64_t arc_batch_slots(const arc_batch* b);
+/* First slot of view v (v = n_views gives the slot count). */
+uint64_t arc_batch_view_first(const arc_batch* b, uint32_t v);
+/* One launch on `stream` (capturable). */
+int arc_batch_trace(arc_batch* b, arc_hit* d_hits, void* stream);
+/* Capture one launch into a hipGraph bound to d_hits; arc_batch_replay launches it. Upload or pool
+ * growth moves the pools: replay then fails until the batch is captured again. */
+int arc_batch_capture(arc_batch* b, arc_hit* d_hits);
+int arc_batch_replay(arc_batch* b, void* stream);
+/* Launch shape of the path kernel, and its autotuner (same candidate grid as arc_autotune). */
+int arc_batch_set_launch_config(arc_batch* b, const arc_launch_config* cfg);
+int arc_batch_get_launch_config(const arc_batch* b, arc_launch_config* cfg);
+int arc_batch_autotune(arc_batch* b, arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
+                       float* best_ms);
+
 
 #ifdef __cplusplus
 }

@@ -76,17 +76,9 @@ uint32_t arc_gen_voxel(const arc_world_host* w, const arc_gen_params* params, ui
                        uint32_t z);
 
 /* ---- rays ------------------------------------------------------------------------------ */
-/* Pinhole camera in the reference convention (types.wgsl:334-345, primary_gbuffer.wgsl:102-115):
- * forward/right/up unit vectors, tan of HALF the VERTICAL fov, aspect = width/height. */
-typedef struct arc_camera {
-  float eye[3];
-  float forward[3];
-  float right[3];
-  float up[3];
-  float tan_half_fov;
-  float aspect;
-  float jitter[2];
-} arc_camera;
+/* arc_camera (declared in arcadia_hip.h) is the pinhole camera of the reference convention
+ * (types.wgsl:334-345, primary_gbuffer.wgsl:102-115): forward/right/up unit vectors, tan of HALF
+ * the VERTICAL fov, aspect = width/height. */
 
 /* Camera looking along yaw/pitch (radians): forward = (cos p sin y, sin p, cos p cos y),
  * right = normalize(cross(forward, +Y)), up = cross(right, forward). */
@@ -105,6 +97,15 @@ int arc_gen_primary_rays(const arc_camera* cam, uint32_t width, uint32_t height,
  * number of rays written; src_index[k] = primary ray index of bounce ray k (may be NULL). */
 int64_t arc_gen_bounce_rays(const arc_ray* primary_rays, const arc_hit* primary_hits, uint64_t n,
                             uint64_t seed, float max_dist_m, arc_ray* rays, uint32_t* src_index);
+
+/* Primary rays of a whole view in padded 8x8-tile order: ceil(w/8)*ceil(h/8)*64 rays, the
+ * out-of-image slots holding the empty ray (t_max = -1). Returns the count or a negative code. */
+int64_t arc_gen_view_rays(const arc_camera* cam, uint32_t width, uint32_t height, float max_dist_m,
+                          arc_ray* rays);
+/* One bounce ray per parent (dense): parent i with a hit gets the diffuse bounce of
+ * arc_gen_bounce_rays hashed with index index_base + i, a miss gets the empty ray. */
+int arc_gen_bounce_rays_dense(const arc_ray* parent_rays, const arc_hit* parent_hits, uint64_t n,
+                              uint64_t index_base, uint64_t seed, float max_dist_m, arc_ray* rays);
 
 /* ---- in-place edits and GPU patches ----------------------------------------------------- */
 /* ChunkEditReport (crates/arcadia-core/src/chunk_tree/edit/set.rs:17-80): dirty index ranges of a
