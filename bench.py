@@ -7,7 +7,10 @@ camera (lod_bias 0, 250 m), then one cosine-weighted diffuse bounce ray per hit 
 (lod_bias 2, ignore_water, 26 m). Exact traversal (no LOD cut-off, no step truncation).
 Rays are resident in HBM before timing; a step is two launches of the traversal kernel.
 
-Prints one JSON line (metric = Grays/s over primary+diffuse rays).
+Prints one JSON line (metric = Grays/s over primary+diffuse rays). A second workload is reported
+beside it under "batched": 64 camera views of uneven sizes (width log-uniform in 320..2560,
+16:9) over a 4096^3 sparse world (16^3 chunks, fBm heightmap terrain in the bottom layer), each
+pixel's primary ray plus 2 diffuse bounces, all in one persistent launch replayed as a hipGraph.
 """
 from __future__ import annotations
 
@@ -94,6 +97,80 @@ def cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads: int, budget_
     }, {"primary": bp, "bounce": bb, "steps_primary": sp, "steps_bounce": sb}
 
 
+def batched_leg(args, threads, stream):
+    """64 uneven views + 2 bounces per hit over a 4096^3 sparse world, one graph replay per frame."""
+    import torch
+
+    import arcadia_amd as A
+
+    t0 = time.time()
+    world = A.HostWorld((16, 16, 16))
+    world.generate(A.gen_params(A.ARC_GEN_SPARSE, seed=args.seed), nthreads=threads)
+    world.pack()
+    pools = world.pools()
+    solid = sum(world.chunk_info(i)["solid"] for i in range(16 ** 3))
+    del world
+    t_build = time.time() - t0
+    rng = np.random.default_rng(args.seed)
+    views = []
+    for _ in range(64):
+        w = int(round(float(np.exp(rng.uniform(np.log(320.0), np.log(2560.0))))))
+        h = max(1, w * 9 // 16)
+        eye = (float(rng.uniform(16.0, 240.0)), float(rng.uniform(16.0, 26.0)), float(rng.uniform(16.0, 240.0)))
+        cam = A.camera_look(eye, float(rng.uniform(0.0, 6.2831853)), float(rng.uniform(-0.6, -0.15)), 70.0, w, h)
+        views.append((cam, w, h, 250.0))
+    ctx = A.Context(0)
+    ctx.upload(*pools)
+    p0, pb = A.TraceParams.exact(0), A.TraceParams.exact(2, ignore_water=True)
+    b = ctx.batch(views, p0, pb, n_bounces=2, bounce_max_dist_m=26.0, seed=args.seed)
+    hits = torch.empty(b.records() * 16, dtype=torch.uint8, device="cuda")
+    sptr = stream.cuda_stream
+    print(f"[bench] batched: world {t_build:.1f}s, {len(pools[2])} nodes, {len(pools[3])} bricks, "
+          f"{b.slots} slots", file=sys.stderr, flush=True)
+    tuned = None
+    if not args.no_autotune:
+        tuned = b.autotune(hits.data_ptr(), sptr, reps=1)
+        print(f"[bench] batched autotuned: {tuned[0]} ({tuned[1]:.2f} ms)", file=sys.stderr, flush=True)
+    b.capture(hits.data_ptr())
+    for _ in range(2):
+        b.replay(sptr)
+    torch.cuda.synchronize()
+    k = max(3, min(args.steps, 10))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        b.replay(sptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / k
+    t = hits.view(torch.float32)[0::4].reshape(3, b.slots)
+    traced = (t != -16.0).sum(dim=1).cpu().numpy().astype(np.int64)
+    n_rays = int(traced.sum())
+    out = {
+        "metric": "Grays/s batched views (primary + 2 diffuse bounces, one persistent launch)",
+        "value": round(n_rays / (ms * 1e-3) / 1e9, 4),
+        "unit": "Grays/s",
+        "ms_per_frame": round(ms, 4),
+        "frames": k,
+        "rays_per_frame": n_rays,
+        "primary_rays": int(traced[0]),
+        "bounce1_rays": int(traced[1]),
+        "bounce2_rays": int(traced[2]),
+        "views": len(views),
+        "slots": int(b.slots),
+        "world": "4096^3 (16^3 chunks), fBm heightmap terrain 32..224 voxels, random mats 1..15",
+        "occupancy": round(solid / float(4096 ** 3), 4),
+        "world_nodes": int(len(pools[2])),
+        "world_bricks": int(len(pools[3])),
+        "launch": b.launch_config(),
+        "autotuned": tuned is not None,
+        "graph": "hipGraph of the single path launch, replayed per frame",
+    }
+    b.close()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +181,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds of CPU work for the baseline sample")
     ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shapes")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -113,6 +191,7 @@ def main() -> int:
         return 2
     if args.profile:
         args.steps, args.warmup, args.no_cpu_baseline, args.no_autotune = min(args.steps, 5), 1, True, True
+        args.no_batched = True
 
     import torch
 
@@ -229,8 +308,13 @@ def main() -> int:
             "avg_launch_ms": round(launch_ms, 4),
         }
         out["cpu_baseline"] = cpu
-    print(json.dumps(out), flush=True)
     ctx.close()
+    del d_prays, d_phits, d_brays, d_bhits
+    if not args.no_batched:
+        out["batched"] = batched_leg(args, threads, stream)
+        print(f"[bench] batched: {out['batched']['value']} Grays/s, {out['batched']['ms_per_frame']} ms/frame",
+              file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
     return 0
 
 
