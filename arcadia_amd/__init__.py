@@ -15,11 +15,14 @@ from .native import (  # noqa: F401
     NODE_DTYPE,
     RAY_DTYPE,
     RESULT_DTYPE,
+    SHARC_PROBE_DTYPE,
+    SHARC_SAMPLE_DTYPE,
     ArcError,
     Batch,
     Context,
     GbufferParams,
     HostWorld,
+    Sharc,
     TraceParams,
     bounce_rays,
     bounce_rays_dense,

@@ -187,6 +187,16 @@ _SIGS = [
     ("arc_batch_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_batch_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig),
                                      C.POINTER(C.c_float)]),
+    ("arc_sharc_create", C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("arc_sharc_destroy", None, [C.c_void_p]),
+    ("arc_sharc_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p)]),
+    ("arc_sharc_clear", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("arc_sharc_read", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("arc_sharc_deposit", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p]),
+    ("arc_sharc_resolve", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("arc_sharc_query", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p,
+                                  C.c_void_p]),
     ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
@@ -714,3 +724,56 @@ class Batch:
         self.ctx._check(lib().arc_batch_autotune(self._h, d_hits, stream or None, reps, C.byref(cfg), C.byref(ms)),
                         "arc_batch_autotune")
         return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}, ms.value
+
+
+SHARC_SAMPLE_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("radiance", "<f4", (3,)),
+                               ("reserved", "<u4")])
+SHARC_PROBE_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("jitter", "<f4", (2,))])
+
+
+class Sharc:
+    """Device SHARC cache (arc_sharc_*); sample / probe buffers are device pointers."""
+
+    def __init__(self, log2_buckets: int = 15, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().arc_sharc_create(device, log2_buckets, C.byref(h))
+        if rc != ARC_OK:
+            raise ArcError(rc, "arc_sharc_create")
+        self._h = h
+        self.log2 = log2_buckets
+        n, k, a, r = C.c_uint32(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        lib().arc_sharc_info(h, C.byref(n), C.byref(k), C.byref(a), C.byref(r))
+        self.slots, self.d_keys, self.d_accum, self.d_resolved = n.value, k.value, a.value, r.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().arc_sharc_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _rc(self, rc, what):
+        if rc != ARC_OK:
+            raise ArcError(rc, what)
+
+    def clear(self, stream: int = 0):
+        self._rc(lib().arc_sharc_clear(self._h, stream or None), "arc_sharc_clear")
+
+    def read(self):
+        """(keys, accum, resolved) copied to host uint32 arrays."""
+        k = np.zeros(self.slots, np.uint32)
+        a = np.zeros(self.slots * 4, np.uint32)
+        r = np.zeros(self.slots * 4, np.uint32)
+        self._rc(lib().arc_sharc_read(self._h, _ptr(k), _ptr(a), _ptr(r)), "arc_sharc_read")
+        return k, a, r
+
+    def deposit(self, d_samples: int, n: int, cam, stream: int = 0):
+        self._rc(lib().arc_sharc_deposit(self._h, d_samples, n, (C.c_float * 3)(*cam), stream or None),
+                 "arc_sharc_deposit")
+
+    def resolve(self, stream: int = 0):
+        self._rc(lib().arc_sharc_resolve(self._h, stream or None), "arc_sharc_resolve")
+
+    def query(self, d_probes: int, n: int, cam, d_out: int, stream: int = 0):
+        self._rc(lib().arc_sharc_query(self._h, d_probes, n, (C.c_float * 3)(*cam), d_out, stream or None),
+                 "arc_sharc_query")

@@ -218,6 +218,44 @@ int arc_primary_gbuffer(arc_ctx* ctx, const arc_trace_params* params, const arc_
                         uint32_t* d_depth_mat, uint32_t* d_norm_motion, uint32_t* d_albedo_ao,
                         void* stream);
 
+/* ---- SHARC radiance cache --------------------------------------------------------------- */
+/* The voxel-surfel hash cache of shaders/common/sharc.wgsl (2^log2_buckets 4-way buckets; the
+ * reference uses 2^15 = 131072 slots, 4.5 MiB; 2^20 buckets = 2^22 slots = 144 MiB). Buffers are
+ * the reference's: keys u32[S], accum u32[4S] ({r, g, b} x1024 fixed point + count), resolved
+ * 16 B slots {key, f16 r|g, f16 b | hist_frames << 16 | stale_age << 24, 0}. */
+typedef struct arc_sharc arc_sharc;
+typedef struct arc_sharc_sample { /* one deposit: surfel position / normal and its radiance */
+  float pos[3];
+  float normal[3];
+  float radiance[3];
+  uint32_t reserved;
+} arc_sharc_sample;
+typedef struct arc_sharc_probe { /* one lookup; jitter (0, 0) = unjittered */
+  float pos[3];
+  float normal[3];
+  float jitter[2];
+} arc_sharc_probe;
+int arc_sharc_create(int device, uint32_t log2_buckets, arc_sharc** out);
+void arc_sharc_destroy(arc_sharc* s);
+/* Slot count and the device buffers (keys, accum, resolved). */
+int arc_sharc_info(const arc_sharc* s, uint32_t* n_slots, uint32_t** keys, uint32_t** accum, void** resolved);
+/* Copy the buffers to host arrays (any may be NULL; blocking). */
+int arc_sharc_read(const arc_sharc* s, uint32_t* keys, uint32_t* accum, uint32_t* resolved);
+/* Zero all three buffers with a fill kernel (capturable; no memset node). */
+int arc_sharc_clear(arc_sharc* s, void* stream);
+/* sharc_make_key + sharc_deposit (sharc.wgsl:71-97, sharc_update.wgsl:129-169) per sample,
+ * unjittered key: claim the surfel's way (existing key first, then the first empty way by CAS),
+ * atomically add u32(clamp(radiance, 0, 32) * 1024) and 1. Which way a new key takes depends on
+ * the order of racing claims; the contents per key do not. */
+int arc_sharc_deposit(arc_sharc* s, const arc_sharc_sample* d_samples, uint64_t n, const float cam_pos[3],
+                      void* stream);
+/* sharc_resolve.wgsl: fold each slot's deposits into its EMA (alpha = 1 / min(hist, 16)), age
+ * untouched surfels and evict them after 64 idle resolves, clear the accumulators. */
+int arc_sharc_resolve(arc_sharc* s, void* stream);
+/* query_sharc (sharc_query.wgsl:20-29): d_out[4 * i ..] = (r, g, b, 1) on a hit, zeros on a miss. */
+int arc_sharc_query(const arc_sharc* s, const arc_sharc_probe* d_queries, uint64_t n, const float cam_pos[3],
+                    float* d_out, void* stream);
+
 /* Kernel launch shape of the persistent ray-queue kernel; 0 fields = autotuned defaults. */
 typedef struct arc_launch_config {
   uint32_t blocks_per_cu;   /* resident workgroups per compute unit */

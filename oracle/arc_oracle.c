@@ -1133,3 +1133,181 @@ void orc_primary_gbuffer(const orc_world* w, const arc_trace_params* p, const ar
   for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
   pthread_mutex_destroy(&j.mu);
 }
+
+/* ---- SHARC cache: shaders/common/sharc.wgsl, sharc_update.wgsl:129-169, sharc_resolve.wgsl,
+ * sharc_query.wgsl. Sequential: deposits claim slots in input order. ----------------------- */
+float orc_f16_to_f32(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu, u;
+  if (e == 0) {
+    if (m == 0) {
+      u = sign;
+    } else { /* subnormal: m * 2^-24 */
+      float f = (float)m * (1.0f / 16777216.0f);
+      memcpy(&u, &f, 4);
+      u |= sign;
+    }
+  } else if (e == 31) {
+    u = sign | 0x7F800000u | (m << 13);
+  } else {
+    u = sign | ((e - 15 + 127) << 23) | (m << 13);
+  }
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+static uint32_t sh_lod(float d) { /* floor(log2(max(d,4)*0.25)) clamped 0..3, exact */
+  float y = fmaxf(d, 4.0f) * 0.25f;
+  int e = 0;
+  frexpf(y, &e); /* y = m * 2^e, m in [0.5, 1) -> floor(log2 y) = e - 1 */
+  e -= 1;
+  return (uint32_t)(e < 0 ? 0 : (e > 3 ? 3 : e));
+}
+
+static uint32_t sh_norm_id(const float n[3]) {
+  float ax = fabsf(n[0]), ay = fabsf(n[1]), az = fabsf(n[2]);
+  if (ax > ay && ax > az) return n[0] > 0.0f ? 0u : 1u;
+  if (ay > az) return n[1] > 0.0f ? 2u : 3u;
+  return n[2] > 0.0f ? 4u : 5u;
+}
+
+static void sh_cross(const float a[3], const float b[3], float o[3]) {
+  float p0 = a[1] * b[2], q0 = a[2] * b[1];
+  float p1 = a[2] * b[0], q1 = a[0] * b[2];
+  float p2 = a[0] * b[1], q2 = a[1] * b[0];
+  o[0] = p0 - q0;
+  o[1] = p1 - q1;
+  o[2] = p2 - q2;
+}
+
+void orc_sharc_make_key(const float pos[3], const float n[3], const float cam[3], float jx, float jy,
+                        uint32_t bucket_mask, uint32_t* bucket, uint32_t* key) {
+  float d[3], a[3];
+  for (int k = 0; k < 3; ++k) d[k] = pos[k] - cam[k];
+  float dd = d[0] * d[0] + d[1] * d[1];
+  dd = dd + d[2] * d[2];
+  uint32_t lod = sh_lod(sqrtf(dd));
+  float s = 0.25f * (float)(1u << lod);
+  for (int k = 0; k < 3; ++k) a[k] = pos[k] - n[k] * 0.02f;
+  if (jx != 0.0f || jy != 0.0f) {
+    float up[3] = {1.0f, 0.0f, 0.0f};
+    if (fabsf(n[2]) < 0.999f) {
+      up[0] = 0.0f;
+      up[2] = 1.0f;
+    }
+    float t[3], b[3];
+    sh_cross(up, n, t);
+    float tl = t[0] * t[0] + t[1] * t[1];
+    tl = sqrtf(tl + t[2] * t[2]);
+    for (int k = 0; k < 3; ++k) t[k] = t[k] / tl;
+    sh_cross(n, t, b);
+    float sc = 0.42f * s;
+    for (int k = 0; k < 3; ++k) {
+      float w = t[k] * jx + b[k] * jy;
+      a[k] = a[k] + w * sc;
+    }
+  }
+  uint32_t c[3];
+  for (int k = 0; k < 3; ++k) c[k] = (uint32_t)((int32_t)floorf(a[k] / s) + 16384);
+  uint32_t nid = sh_norm_id(n);
+  uint32_t bk = pcg_hash(c[0] * 73856093u ^ c[1] * 19349663u ^ c[2] * 83492791u ^ nid * 2654435761u ^ lod * 38069u) &
+                bucket_mask;
+  *bucket = bk;
+  *key = pcg_hash(bk ^ (c[0] * 1013904223u) ^ (c[1] * 1664525u) ^ (c[2] * 2891336453u) ^ (nid << 20) ^ (lod << 24)) |
+         0x80000000u;
+}
+
+void orc_sharc_deposit(uint32_t log2_buckets, uint32_t* keys, uint32_t* accum, const arc_sharc_sample* s, uint64_t n,
+                       const float cam[3]) {
+  uint32_t mask = (1u << log2_buckets) - 1u;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t bk, key, slot = 0xFFFFFFFFu;
+    orc_sharc_make_key(s[i].pos, s[i].normal, cam, 0.0f, 0.0f, mask, &bk, &key);
+    for (uint32_t w = 0; w < 4 && slot == 0xFFFFFFFFu; ++w)
+      if (keys[bk * 4 + w] == key) slot = bk * 4 + w;
+    for (uint32_t w = 0; w < 4 && slot == 0xFFFFFFFFu; ++w)
+      if (keys[bk * 4 + w] == 0u) {
+        keys[bk * 4 + w] = key;
+        slot = bk * 4 + w;
+      }
+    if (slot == 0xFFFFFFFFu) continue;
+    for (int c = 0; c < 3; ++c) {
+      float v = clampf_(s[i].radiance[c], 0.0f, 32.0f) * 1024.0f;
+      accum[slot * 4 + c] += f2u(v);
+    }
+    accum[slot * 4 + 3] += 1u;
+  }
+}
+
+void orc_sharc_resolve(uint32_t n_slots, uint32_t* keys, uint32_t* accum, uint32_t* resolved) {
+  for (uint32_t slot = 0; slot < n_slots; ++slot) {
+    uint32_t key = keys[slot], *rs = resolved + 4 * slot, *ac = accum + 4 * slot;
+    if (key == 0u) {
+      rs[0] = rs[1] = rs[2] = rs[3] = 0u;
+      continue;
+    }
+    int prev_valid = rs[0] == key;
+    float rad[3] = {0.0f, 0.0f, 0.0f};
+    uint32_t hist = 0, age = 0;
+    if (prev_valid) {
+      rad[0] = orc_f16_to_f32((uint16_t)(rs[1] & 0xFFFFu));
+      rad[1] = orc_f16_to_f32((uint16_t)(rs[1] >> 16));
+      rad[2] = orc_f16_to_f32((uint16_t)(rs[2] & 0xFFFFu));
+      hist = (rs[2] >> 16) & 0xFFu;
+      age = (rs[2] >> 24) & 0xFFu;
+    }
+    if (ac[3] > 0u) {
+      float inv = 1.0f / ((float)ac[3] * 1024.0f);
+      hist = hist + 1u < 255u ? hist + 1u : 255u;
+      float alpha = 1.0f / fminf((float)hist, 16.0f);
+      for (int c = 0; c < 3; ++c) {
+        float smp = (float)ac[c] * inv;
+        rad[c] = mix_(rad[c], smp, alpha);
+      }
+      age = 0;
+      ac[0] = ac[1] = ac[2] = ac[3] = 0u;
+    } else {
+      age = age + 1u;
+      if (!prev_valid || age > 64u) {
+        keys[slot] = 0u;
+        rs[0] = rs[1] = rs[2] = rs[3] = 0u;
+        continue;
+      }
+    }
+    float c0 = clampf_(rad[0], 0.0f, 32.0f), c1 = clampf_(rad[1], 0.0f, 32.0f), c2 = clampf_(rad[2], 0.0f, 32.0f);
+    rs[0] = key;
+    rs[1] = (uint32_t)orc_f32_to_f16(c0) | ((uint32_t)orc_f32_to_f16(c1) << 16);
+    rs[2] = (uint32_t)orc_f32_to_f16(c2) | ((hist & 0xFFu) << 16) | ((age & 0xFFu) << 24);
+    rs[3] = 0u;
+  }
+}
+
+static void sh_lookup(const uint32_t* resolved, uint32_t bk, uint32_t key, float out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0.0f;
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t* rs = resolved + 4 * (bk * 4 + w);
+    if (rs[0] == key) {
+      out[0] = orc_f16_to_f32((uint16_t)(rs[1] & 0xFFFFu));
+      out[1] = orc_f16_to_f32((uint16_t)(rs[1] >> 16));
+      out[2] = orc_f16_to_f32((uint16_t)(rs[2] & 0xFFFFu));
+      out[3] = 1.0f;
+      return;
+    }
+  }
+}
+
+void orc_sharc_query(uint32_t log2_buckets, const uint32_t* resolved, const arc_sharc_probe* q, uint64_t n,
+                     const float cam[3], float* out) {
+  uint32_t mask = (1u << log2_buckets) - 1u;
+  for (uint64_t i = 0; i < n; ++i) {
+    uint32_t bk, key;
+    float* o = out + 4 * i;
+    if (q[i].jitter[0] != 0.0f || q[i].jitter[1] != 0.0f) {
+      orc_sharc_make_key(q[i].pos, q[i].normal, cam, q[i].jitter[0], q[i].jitter[1], mask, &bk, &key);
+      sh_lookup(resolved, bk, key, o);
+      if (o[3] > 0.5f) continue;
+    }
+    orc_sharc_make_key(q[i].pos, q[i].normal, cam, 0.0f, 0.0f, mask, &bk, &key);
+    sh_lookup(resolved, bk, key, o);
+  }
+}

@@ -71,6 +71,18 @@ void orc_primary_gbuffer(const orc_world* w, const arc_trace_params* p, const ar
 /* f32 -> f16 bits, round to nearest even (the conversion of pack2x16float). */
 uint16_t orc_f32_to_f16(float x);
 
+/* SHARC cache (shaders/common/sharc.wgsl, sharc_update.wgsl:129-169, sharc_resolve.wgsl,
+ * sharc_query.wgsl): keys u32[S], accum u32[4S], resolved u32[4S], S = 4 << log2_buckets.
+ * Deposits claim slots sequentially in input order. */
+float orc_f16_to_f32(uint16_t h);
+void orc_sharc_make_key(const float pos[3], const float n[3], const float cam[3], float jx, float jy,
+                        uint32_t bucket_mask, uint32_t* bucket, uint32_t* key);
+void orc_sharc_deposit(uint32_t log2_buckets, uint32_t* keys, uint32_t* accum, const arc_sharc_sample* s, uint64_t n,
+                       const float cam[3]);
+void orc_sharc_resolve(uint32_t n_slots, uint32_t* keys, uint32_t* accum, uint32_t* resolved);
+void orc_sharc_query(uint32_t log2_buckets, const uint32_t* resolved, const arc_sharc_probe* q, uint64_t n,
+                     const float cam[3], float* out);
+
 uint32_t orc_query_voxel(const orc_world* w, uint32_t x, uint32_t y, uint32_t z);
 /* Dense material grid (x fastest) of the whole world, size (gx*256)*(gy*256)*(gz*256). */
 void orc_decode_dense(const orc_world* w, uint8_t* dense, int nthreads);

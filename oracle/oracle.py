@@ -66,6 +66,14 @@ def lib() -> C.CDLL:
         h.orc_brute_dense.argtypes = [P, P, P, C.c_uint64, C.c_int, C.c_double, P, C.c_int]
         h.orc_primary_gbuffer.argtypes = [P, P, P, P, P, P, P, C.c_int]
         h.orc_primary_gbuffer.restype = None
+        h.orc_f16_to_f32.argtypes = [C.c_uint16]
+        h.orc_f16_to_f32.restype = C.c_float
+        h.orc_sharc_make_key.argtypes = [P, P, P, C.c_float, C.c_float, C.c_uint32, P, P]
+        h.orc_sharc_deposit.argtypes = [C.c_uint32, P, P, P, C.c_uint64, P]
+        h.orc_sharc_resolve.argtypes = [C.c_uint32, P, P, P]
+        h.orc_sharc_query.argtypes = [C.c_uint32, P, P, C.c_uint64, P, P]
+        for f in (h.orc_sharc_make_key, h.orc_sharc_deposit, h.orc_sharc_resolve, h.orc_sharc_query):
+            f.restype = None
         h.orc_f32_to_f16.argtypes = [C.c_float]
         h.orc_f32_to_f16.restype = C.c_uint16
         for f in (h.orc_trace_world, h.orc_trace_chunk, h.orc_trace_meters, h.orc_compact, h.orc_decode_dense,
@@ -184,3 +192,42 @@ def primary_gbuffer(world: World, params, g, palette: np.ndarray, nthreads: int 
 
 def f32_to_f16_bits(x: float) -> int:
     return int(lib().orc_f32_to_f16(float(x)))
+
+
+SHARC_SAMPLE_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("radiance", "<f4", (3,)),
+                               ("reserved", "<u4")])
+SHARC_PROBE_DTYPE = np.dtype([("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("jitter", "<f4", (2,))])
+
+
+class Sharc:
+    """Sequential SHARC cache (keys, accum, resolved as numpy arrays)."""
+
+    def __init__(self, log2_buckets: int = 15):
+        self.log2 = log2_buckets
+        self.slots = 4 << log2_buckets
+        self.keys = np.zeros(self.slots, np.uint32)
+        self.accum = np.zeros(self.slots * 4, np.uint32)
+        self.resolved = np.zeros(self.slots * 4, np.uint32)
+
+    def make_key(self, pos, normal, cam, jitter=(0.0, 0.0)):
+        b, k = C.c_uint32(), C.c_uint32()
+        lib().orc_sharc_make_key((C.c_float * 3)(*pos), (C.c_float * 3)(*normal), (C.c_float * 3)(*cam), jitter[0],
+                                 jitter[1], (1 << self.log2) - 1, C.byref(b), C.byref(k))
+        return b.value, k.value
+
+    def deposit(self, samples: np.ndarray, cam) -> None:
+        s = np.ascontiguousarray(samples, SHARC_SAMPLE_DTYPE)
+        if len(s):
+            lib().orc_sharc_deposit(self.log2, self.keys.ctypes.data, self.accum.ctypes.data, s.ctypes.data, len(s),
+                                    (C.c_float * 3)(*cam))
+
+    def resolve(self) -> None:
+        lib().orc_sharc_resolve(self.slots, self.keys.ctypes.data, self.accum.ctypes.data, self.resolved.ctypes.data)
+
+    def query(self, probes: np.ndarray, cam) -> np.ndarray:
+        q = np.ascontiguousarray(probes, SHARC_PROBE_DTYPE)
+        out = np.zeros((len(q), 4), np.float32)
+        if len(q):
+            lib().orc_sharc_query(self.log2, self.resolved.ctypes.data, q.ctypes.data, len(q), (C.c_float * 3)(*cam),
+                                  out.ctypes.data)
+        return out
