@@ -22,11 +22,16 @@ from .native import (  # noqa: F401
     Context,
     GbufferParams,
     HostWorld,
+    Nrc,
+    NrcParams,
+    PATH_SAMPLE_DTYPE,
+    ShadeParams,
     Sharc,
     TraceParams,
     bounce_rays,
     bounce_rays_dense,
     camera_look,
+    estimate,
     gen_params,
     lib,
     lod_payload,

@@ -15,6 +15,7 @@
 #include "gbuffer.h"
 #include "patch.h"
 #include "patch_plan.h"
+#include "pathtrace.h"
 #include "trace_launch.h"
 
 struct arc_ctx {
@@ -627,7 +628,7 @@ int arc_batch_create(arc_ctx* c, const arc_view* views, uint32_t n_views, const 
   if (!valid_params(primary) || !valid_params(bounce) || primary->semantics != ARC_SEM_WORLD ||
       bounce->semantics != ARC_SEM_WORLD)
     return fail(c, ARC_ERR_INVALID, "batches trace world semantics");
-  if (n_bounces > 3) return fail(c, ARC_ERR_INVALID, "at most 3 bounces");
+  if (n_bounces > 4) return fail(c, ARC_ERR_INVALID, "at most 4 bounces");
   std::vector<arcdev::ViewDev> hv(n_views);
   std::vector<uint32_t> first(n_views + 1);
   uint64_t items = 0;
@@ -673,6 +674,12 @@ int arc_batch_create(arc_ctx* c, const arc_view* views, uint32_t n_views, const 
 }
 
 uint64_t arc_batch_slots(const arc_batch* b) { return b ? b->n_items : 0; }
+
+int arc_batch_set_seed(arc_batch* b, uint64_t seed) {
+  if (!b) return ARC_ERR_INVALID;
+  b->seed = seed;
+  return ARC_OK;
+}
 
 uint64_t arc_batch_view_first(const arc_batch* b, uint32_t v) {
   return (b && v <= b->n_views) ? b->first[v] : 0;
@@ -785,6 +792,42 @@ int arc_batch_autotune(arc_batch* b, arc_hit* d_hits, void* stream, uint32_t rep
   if (best) *best = win;
   if (best_ms) *best_ms = win_ms;
   return ARC_OK;
+}
+
+int arc_batch_shade(arc_batch* b, const arc_hit* d_hits, const arc_shade_params* sp, arc_path_sample* d_samples,
+                    float* d_radiance, double* d_accum, void* stream) {
+  if (!b || !sp) return ARC_ERR_INVALID;
+  arc_ctx* c = b->ctx;
+  if (!d_hits) return fail(c, ARC_ERR_INVALID, "null hit buffer");
+  HIP_TRY(c, hipSetDevice(c->device));
+  arcdev::ShadeArgs a{};
+  a.views = b->d_views;
+  a.view_first = b->d_first;
+  a.n_views = b->n_views;
+  a.n_items = (uint32_t)b->n_items;
+  a.n_bounces = b->n_bounces;
+  a.bounce_tmax = b->bounce_tmax;
+  a.seed = b->seed;
+  a.hits = reinterpret_cast<const uint4*>(d_hits);
+  a.palette = c->d_palette;
+  std::memcpy(a.sun, sp->sun_dir, sizeof a.sun);
+  a.emission_scale = sp->emission_scale;
+  a.sky_scale = sp->sky_scale;
+  a.samples = d_samples;
+  a.radiance = d_radiance;
+  a.accum = d_accum;
+  hipError_t e = arcdev::launch_shade(a, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "shade launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
+int arc_estimate(const arc_path_sample* d_samples, const float* d_cache, uint64_t n, uint32_t mode, float* d_out,
+                 void* stream) {
+  if (n == 0) return ARC_OK;
+  if (!d_samples || !d_out || (mode == 1 && !d_cache) || mode > 1 || n > 0xFFFFFFFFull) return ARC_ERR_INVALID;
+  return arcdev::launch_estimate(d_samples, d_cache, (uint32_t)n, mode, d_out, (hipStream_t)stream) == hipSuccess
+             ? ARC_OK
+             : ARC_ERR_HIP;
 }
 
 void arc_batch_destroy(arc_batch* b) {

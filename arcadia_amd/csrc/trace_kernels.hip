@@ -43,7 +43,7 @@ enum : uint32_t {
   F_LIVE = 6,    // 1 bit: ray in flight
   F_HIT = 7,     // 1 bit: finished with a hit
   F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
-  F_DEPTH = 9,   // 2 bits: path kernel: 0 = primary ray, d = d-th bounce
+  F_DEPTH = 9,   // 3 bits: path kernel: 0 = primary ray, d = d-th bounce
 };
 
 __device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
@@ -617,7 +617,7 @@ __device__ __forceinline__ void path_start(const DevWorld& w, const DevParams& p
 // Writes the finished ray's record, then starts its bounce or closes the slot.
 __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& pb, const PathCtl& pc, uint4* out,
                                             Lane& L, float& tmax, uint4* C) {
-  const uint32_t depth = fget(L.st, F_DEPTH, 2);
+  const uint32_t depth = fget(L.st, F_DEPTH, 3);
   const uint32_t steps = L.step > 0xFFFFu ? 0xFFFFu : L.step;
   const bool hit = fget(L.st, F_HIT, 1) != 0u;
   uint4 v;
@@ -644,7 +644,7 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
     arc_ray r;
     arcmath::bounce_ray(ov, dr, exact_t, face, pc.seed, (uint64_t)depth * pc.n_items + item, pc.bounce_tmax, &r);
     ray_init<ARC_SEM_WORLD>(w, pb, r, L, C);
-    L.st = fput(L.st, F_DEPTH, 2, depth + 1u);
+    L.st = fput(L.st, F_DEPTH, 3, depth + 1u);
     tmax = pc.bounce_tmax;
   } else {
     for (uint32_t d = depth + 1u; d <= pc.n_bounces; ++d) out[(size_t)d * pc.n_items + item] = empty_record();
@@ -701,7 +701,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       continue;
     }
     if (drained && nidle == 64u) break;
-    if (!idle_lane) iteration<ARC_SEM_WORLD>(w, sel_params(fget(L.st, F_DEPTH, 2) != 0u, pb, p0), L, C);
+    if (!idle_lane) iteration<ARC_SEM_WORLD>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C);
   }
   __syncthreads();
   if (threadIdx.x == 0) {

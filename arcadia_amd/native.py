@@ -130,6 +130,28 @@ class EditReport(C.Structure):
         return self.min_dirty_brick <= self.max_dirty_brick
 
 
+class ShadeParams(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("emission_scale", C.c_float), ("sky_scale", C.c_float),
+                ("reserved", C.c_uint32 * 3)]
+
+    @classmethod
+    def default(cls) -> "ShadeParams":
+        s = np.array([0.35, 0.80, 0.45])
+        s /= np.linalg.norm(s)
+        return cls((C.c_float * 3)(*s.astype(np.float32).tolist()), 1.0, 1.0, (C.c_uint32 * 3)())
+
+
+class NrcParams(C.Structure):
+    _fields_ = [("bbox_min", C.c_float * 3), ("bbox_size", C.c_float), ("log2_table", C.c_uint32),
+                ("base_res", C.c_float), ("max_res", C.c_float), ("lr_grid", C.c_float), ("lr_mlp", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
+
+    @classmethod
+    def default(cls, bbox_min=(0.0, 0.0, 0.0), bbox_size=64.0, log2_table=19, seed=1) -> "NrcParams":
+        return cls((C.c_float * 3)(*bbox_min), bbox_size, log2_table, 16.0, 2048.0, 1e-2, 1e-3, 0.9, 0.99, 1e-10,
+                   seed)
+
+
 class GbufferParams(C.Structure):
     """arc_gbuffer_params: camera basis, jitter, previous view-projection (column-major)."""
     _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3),
@@ -197,6 +219,21 @@ _SIGS = [
     ("arc_sharc_resolve", C.c_int, [C.c_void_p, C.c_void_p]),
     ("arc_sharc_query", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p,
                                   C.c_void_p]),
+    ("arc_batch_set_seed", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("arc_batch_shade", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(ShadeParams), C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
+    ("arc_estimate", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("arc_nrc_create", C.c_int, [C.c_int, C.POINTER(NrcParams), C.POINTER(C.c_void_p)]),
+    ("arc_nrc_destroy", None, [C.c_void_p]),
+    ("arc_nrc_reset", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("arc_nrc_infer", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
+    ("arc_nrc_train", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("arc_nrc_backward", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("arc_nrc_zero_grads", C.c_int, [C.c_void_p]),
+    ("arc_nrc_loss", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    ("arc_nrc_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_float),
+                               C.POINTER(C.c_uint32)]),
+    ("arc_nrc_params_io", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
@@ -719,6 +756,14 @@ class Batch:
         self.ctx._check(lib().arc_batch_get_launch_config(self._h, C.byref(cfg)), "arc_batch_get_launch_config")
         return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}
 
+    def set_seed(self, seed: int) -> None:
+        self.ctx._check(lib().arc_batch_set_seed(self._h, seed), "arc_batch_set_seed")
+
+    def shade(self, d_hits: int, params: "ShadeParams", d_samples: int = 0, d_radiance: int = 0, d_accum: int = 0,
+              stream: int = 0) -> None:
+        self.ctx._check(lib().arc_batch_shade(self._h, d_hits, C.byref(params), d_samples or None, d_radiance or None,
+                                              d_accum or None, stream or None), "arc_batch_shade")
+
     def autotune(self, d_hits: int, stream: int = 0, reps: int = 3):
         cfg, ms = LaunchConfig(), C.c_float()
         self.ctx._check(lib().arc_batch_autotune(self._h, d_hits, stream or None, reps, C.byref(cfg), C.byref(ms)),
@@ -777,3 +822,79 @@ class Sharc:
     def query(self, d_probes: int, n: int, cam, d_out: int, stream: int = 0):
         self._rc(lib().arc_sharc_query(self._h, d_probes, n, (C.c_float * 3)(*cam), d_out, stream or None),
                  "arc_sharc_query")
+
+
+PATH_SAMPLE_DTYPE = np.dtype([("pos", "<f4", (3,)), ("flags", "<u4"), ("dir", "<f4", (3,)), ("pad0", "<u4"),
+                              ("le0", "<f4", (3,)), ("pad1", "<u4"), ("a0", "<f4", (3,)), ("pad2", "<u4"),
+                              ("base", "<f4", (3,)), ("pad3", "<u4"), ("target", "<f4", (3,)), ("pad4", "<u4")])
+PS_RAY, PS_X0_HIT, PS_X1_HIT = 1, 2, 4
+
+
+def estimate(d_samples: int, n: int, d_cache: int, mode: int, d_out: int, stream: int = 0) -> None:
+    rc = lib().arc_estimate(d_samples, d_cache or None, n, mode, d_out, stream or None)
+    if rc != ARC_OK:
+        raise ArcError(rc, "arc_estimate")
+
+
+class Nrc:
+    """Neural radiance cache (arc_nrc_*)."""
+
+    def __init__(self, params: NrcParams, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().arc_nrc_create(device, C.byref(params), C.byref(h))
+        if rc != ARC_OK:
+            raise ArcError(rc, "arc_nrc_create")
+        self._h = h
+        self.params = params
+        gp, mp = C.c_uint64(), C.c_uint32()
+        sc, dn = (C.c_float * 16)(), (C.c_uint32 * 16)()
+        lib().arc_nrc_info(h, C.byref(gp), C.byref(mp), sc, dn)
+        self.grid_params, self.mlp_params = gp.value, mp.value
+        self.scale = np.array(sc[:], np.float32)
+        self.dense = np.array(dn[:], np.uint32)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().arc_nrc_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def _rc(self, rc, what):
+        if rc != ARC_OK:
+            raise ArcError(rc, what)
+
+    def reset(self, seed: int):
+        self._rc(lib().arc_nrc_reset(self._h, seed), "arc_nrc_reset")
+
+    def infer(self, d_samples: int, n: int, d_out: int, stream: int = 0):
+        self._rc(lib().arc_nrc_infer(self._h, d_samples, n, d_out, stream or None), "arc_nrc_infer")
+
+    def train(self, d_samples: int, n: int, stream: int = 0):
+        self._rc(lib().arc_nrc_train(self._h, d_samples, n, stream or None), "arc_nrc_train")
+
+    def backward(self, d_samples: int, n: int, stream: int = 0):
+        self._rc(lib().arc_nrc_backward(self._h, d_samples, n, stream or None), "arc_nrc_backward")
+
+    def zero_grads(self):
+        self._rc(lib().arc_nrc_zero_grads(self._h), "arc_nrc_zero_grads")
+
+    def loss(self, reset: bool = True) -> float:
+        v = C.c_float()
+        self._rc(lib().arc_nrc_loss(self._h, C.byref(v), int(reset)), "arc_nrc_loss")
+        return v.value
+
+    def get(self):
+        """(grid, mlp, grid_grad, mlp_grad) FP32 host copies."""
+        g = np.zeros(self.grid_params, np.float32)
+        m = np.zeros(self.mlp_params, np.float32)
+        gg = np.zeros(self.grid_params, np.float32)
+        mg = np.zeros(self.mlp_params, np.float32)
+        self._rc(lib().arc_nrc_params_io(self._h, _ptr(g), _ptr(m), _ptr(gg), _ptr(mg), 0), "arc_nrc_params_io")
+        return g, m, gg, mg
+
+    def set(self, grid: np.ndarray = None, mlp: np.ndarray = None):
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        self._rc(lib().arc_nrc_params_io(self._h, _ptr(g) if g is not None else None,
+                                         _ptr(m) if m is not None else None, None, None, 1), "arc_nrc_params_io")

@@ -148,6 +148,7 @@ typedef struct arc_camera {
 } arc_camera;
 
 typedef struct arc_ctx arc_ctx;
+typedef struct arc_batch arc_batch; /* batched views, see arc_batch_create */
 
 /* ---- context --------------------------------------------------------------------------- */
 /* Replaces the wgpu device/queue owned by Renderer (crates/arcadia-render/src/context.rs:25-146). */
@@ -217,6 +218,84 @@ typedef struct arc_gbuffer_params {
 int arc_primary_gbuffer(arc_ctx* ctx, const arc_trace_params* params, const arc_gbuffer_params* g,
                         uint32_t* d_depth_mat, uint32_t* d_norm_motion, uint32_t* d_albedo_ao,
                         void* stream);
+
+/* ---- path shading (radiance cache workloads) -------------------------------------------- */
+/* Light transport of the radiance-cache workloads (no reference counterpart; assembled from the
+ * reference's materials and sky): Lambertian palette albedo, palette emission x emission_scale,
+ * escaped rays see eval_sky_radiance (types.wgsl:500-523) x sky_scale. */
+typedef struct arc_shade_params {
+  float sun_dir[3];      /* unit vector towards the sun */
+  float emission_scale;
+  float sky_scale;
+  uint32_t reserved[3];
+} arc_shade_params;
+/* One slot of a shaded path launch. x0 = primary hit, x1 = hit of the first bounce. The path's
+ * radiance is le0 + a0 * (base + target) (x1 hit), le0 + a0 * base (bounce escaped: base = sky),
+ * le0 (primary miss: le0 = sky, a0 = 0). A radiance cache at x1 replaces `target`, the rest of the
+ * path beyond x1 (its emission excluded, as SHARC stores it, sharc_update.wgsl:63-64). */
+#define ARC_PS_RAY 1u     /* the slot has a primary ray */
+#define ARC_PS_X0_HIT 2u
+#define ARC_PS_X1_HIT 4u  /* bits 8..15: face code of x1 */
+typedef struct arc_path_sample {
+  float pos[3];    /* x1, meters */
+  uint32_t flags;
+  float dir[3];    /* direction x0 -> x1 */
+  uint32_t pad0;
+  float le0[3];
+  uint32_t pad1;
+  float a0[3];
+  uint32_t pad2;
+  float base[3];   /* Le1, or the sky seen by the first bounce */
+  uint32_t pad3;
+  float target[3]; /* a1 (Le2 + a2 (...)): the path's radiance beyond x1 */
+  uint32_t pad4;
+} arc_path_sample;
+/* Shade the records of arc_batch_trace (the batch's current seed): per slot the path sample,
+ * its radiance (3 floats) and d_accum += radiance (3 doubles). Any output may be NULL. */
+int arc_batch_shade(arc_batch* b, const arc_hit* d_hits, const arc_shade_params* params, arc_path_sample* d_samples,
+                    float* d_radiance, double* d_accum, void* stream);
+/* Per slot: le0 + a0 * (base + max(cache, 0)) with cache = d_cache[4 i .. 4 i + 2] where x1 was
+ * hit (mode 1), or le0 + a0 * base (mode 0, no cache). 3 floats per slot into d_out. */
+int arc_estimate(const arc_path_sample* d_samples, const float* d_cache, uint64_t n, uint32_t mode, float* d_out,
+                 void* stream);
+
+/* ---- neural radiance cache ---------------------------------------------------------------- */
+/* Learned radiance cache (no reference counterpart): 16-level hash-grid encoding of the
+ * position (2 FP16 features per level, 2^log2_table entries per level) + normal + view
+ * direction -> MLP 3 x 64 ReLU -> RGB, FP16 weights with FP32 accumulation on the matrix cores,
+ * trained online with a relative L2 loss (mean over RGB of (y - t)^2 / (y^2 + 0.01), denominator held
+ * constant) and Adam. It predicts `target` of arc_path_sample
+ * (the radiance beyond x1) from x1's position, normal and the incoming direction. */
+typedef struct arc_nrc arc_nrc;
+typedef struct arc_nrc_params {
+  float bbox_min[3];   /* cube enclosing the scene, meters */
+  float bbox_size;
+  uint32_t log2_table; /* entries per level (10..24) */
+  float base_res;      /* coarsest level resolution (cells per bbox edge) */
+  float max_res;       /* finest level resolution */
+  float lr_grid, lr_mlp;
+  float beta1, beta2, eps;
+  uint64_t seed;       /* initialization */
+} arc_nrc_params;
+int arc_nrc_create(int device, const arc_nrc_params* params, arc_nrc** out);
+void arc_nrc_destroy(arc_nrc* r);
+/* Re-initialize weights and table (seeded), clear moments and gradients. */
+int arc_nrc_reset(arc_nrc* r, uint64_t seed);
+/* Prediction per sample: d_out[4 i ..] = (r, g, b, 1 if x1 was hit else 0). */
+int arc_nrc_infer(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, float* d_out, void* stream);
+/* One training step on n samples (samples without an x1 hit carry no loss): gradients of the
+ * summed loss, then Adam with the gradients scaled by 1/n. */
+int arc_nrc_train(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, void* stream);
+/* Gradients only (accumulated into the FP32 gradient buffers; no update). */
+int arc_nrc_backward(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, void* stream);
+int arc_nrc_zero_grads(arc_nrc* r);
+/* Summed loss since the last reset (blocking). */
+int arc_nrc_loss(arc_nrc* r, float* loss_sum, int reset);
+/* Parameter counts and the level resolutions / dense-indexing flags. */
+int arc_nrc_info(const arc_nrc* r, uint64_t* grid_params, uint32_t* mlp_params, float scale[16], uint32_t dense[16]);
+/* FP32 masters (table [16][2^T][2], MLP W1[64][48] W2[64][64] W3[64][64] W4[16][64]) and gradients
+ * to host arrays (set = 0; any may be NULL), or load masters from them (set = 1). Blocking. */
+int arc_nrc_params_io(arc_nrc* r, float* grid, float* mlp, float* grid_grad, float* mlp_grad, int set);
 
 /* ---- SHARC radiance cache --------------------------------------------------------------- */
 /* The voxel-surfel hash cache of shaders/common/sharc.wgsl (2^log2_buckets 4-way buckets; the
@@ -293,11 +372,13 @@ typedef struct arc_view {
   float max_dist_m;    /* primary ray length (250 m in the reference) */
   uint32_t reserved;
 } arc_view;
-typedef struct arc_batch arc_batch;
+/* n_bounces 0..4. */
 int arc_batch_create(arc_ctx* ctx, const arc_view* views, uint32_t n_views, const arc_trace_params* primary,
                      const arc_trace_params* bounce, uint32_t n_bounces, float bounce_max_dist_m, uint64_t seed,
                      arc_batch** out);
 void arc_batch_destroy(arc_batch* b);
+/* Seed of the bounce hash for the following launches (a captured graph keeps its own). */
+int arc_batch_set_seed(arc_batch* b, uint64_t seed);
 /* Pixel slots (padded) of the batch; the hit buffer holds (1 + n_bounces) * slots records. */
 uint64_t arc_batch_slots(const arc_batch* b);
 /* First slot of view v (v = n_views gives the slot count). */
