@@ -232,6 +232,51 @@ __global__ __launch_bounds__(256) void k_sharc_query(const arc_sharc_probe* __re
   out[i] = lookup(resolved, make_key(e.pos, e.normal, cam, 0.0f, 0.0f, bucket_mask));
 }
 
+// Deposits / queries straight from shaded path samples (x1 position, normal from its face code,
+// radiance = the path's radiance beyond x1); samples without an x1 hit are skipped.
+__device__ __forceinline__ bool path_x1(const arc_path_sample& ps, float n[3]) {
+  n[0] = n[1] = n[2] = 0.0f;
+  if (!(ps.flags & ARC_PS_X1_HIT)) return false;
+  const uint32_t face = (ps.flags >> 8) & 0xFFu;
+  if (face < 6u) n[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_sharc_deposit_paths(const arc_path_sample* __restrict__ s, uint32_t n,
+                                                             float cx, float cy, float cz, uint32_t bucket_mask,
+                                                             uint32_t* __restrict__ keys, uint32_t* __restrict__ accum) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const arc_path_sample ps = s[i];
+  float nrm[3];
+  if (!path_x1(ps, nrm)) return;
+  const float cam[3] = {cx, cy, cz};
+  const Key k = make_key(ps.pos, nrm, cam, 0.0f, 0.0f, bucket_mask);
+  const uint32_t slot = claim_slot(keys, k);
+  if (slot == kNoSlot) return;
+  uint32_t* a = accum + 4u * slot;
+  for (int c = 0; c < 3; ++c) {
+    const float v = fminf(fmaxf(ps.target[c], 0.0f), kMaxRadiance) * kFixedScale;
+    atomicAdd(a + c, (uint32_t)v);
+  }
+  atomicAdd(a + 3, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_sharc_query_paths(const arc_path_sample* __restrict__ s, uint32_t n, float cx,
+                                                           float cy, float cz, uint32_t bucket_mask,
+                                                           const uint4* __restrict__ resolved, float4* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const arc_path_sample ps = s[i];
+  float nrm[3];
+  if (!path_x1(ps, nrm)) {
+    out[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const float cam[3] = {cx, cy, cz};
+  out[i] = lookup(resolved, make_key(ps.pos, nrm, cam, 0.0f, 0.0f, bucket_mask));
+}
+
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0u;
@@ -340,6 +385,27 @@ int arc_sharc_query(const arc_sharc* s, const arc_sharc_probe* d_queries, uint64
   if (hipSetDevice(s->device) != hipSuccess) return ARC_ERR_HIP;
   hipLaunchKernelGGL(k_sharc_query, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, d_queries,
                      (uint32_t)n, cam_pos[0], cam_pos[1], cam_pos[2], s->buckets - 1u, s->resolved,
+                     reinterpret_cast<float4*>(d_out));
+  return hipGetLastError() == hipSuccess ? ARC_OK : ARC_ERR_HIP;
+}
+
+int arc_sharc_deposit_paths(arc_sharc* s, const arc_path_sample* d_samples, uint64_t n, const float cam_pos[3],
+                            void* stream) {
+  if (!s || !cam_pos || (n && !d_samples) || n > 0x7FFFFFFFull) return ARC_ERR_INVALID;
+  if (n == 0) return ARC_OK;
+  if (hipSetDevice(s->device) != hipSuccess) return ARC_ERR_HIP;
+  hipLaunchKernelGGL(k_sharc_deposit_paths, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     d_samples, (uint32_t)n, cam_pos[0], cam_pos[1], cam_pos[2], s->buckets - 1u, s->keys, s->accum);
+  return hipGetLastError() == hipSuccess ? ARC_OK : ARC_ERR_HIP;
+}
+
+int arc_sharc_query_paths(const arc_sharc* s, const arc_path_sample* d_samples, uint64_t n, const float cam_pos[3],
+                          float* d_out, void* stream) {
+  if (!s || !cam_pos || (n && (!d_samples || !d_out)) || n > 0x7FFFFFFFull) return ARC_ERR_INVALID;
+  if (n == 0) return ARC_OK;
+  if (hipSetDevice(s->device) != hipSuccess) return ARC_ERR_HIP;
+  hipLaunchKernelGGL(k_sharc_query_paths, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     d_samples, (uint32_t)n, cam_pos[0], cam_pos[1], cam_pos[2], s->buckets - 1u, s->resolved,
                      reinterpret_cast<float4*>(d_out));
   return hipGetLastError() == hipSuccess ? ARC_OK : ARC_ERR_HIP;
 }

@@ -217,6 +217,9 @@ _SIGS = [
     ("arc_sharc_read", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("arc_sharc_deposit", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p]),
     ("arc_sharc_resolve", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("arc_sharc_deposit_paths", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p]),
+    ("arc_sharc_query_paths", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p,
+                                        C.c_void_p]),
     ("arc_sharc_query", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p,
                                   C.c_void_p]),
     ("arc_batch_set_seed", C.c_int, [C.c_void_p, C.c_uint64]),
@@ -818,6 +821,14 @@ class Sharc:
 
     def resolve(self, stream: int = 0):
         self._rc(lib().arc_sharc_resolve(self._h, stream or None), "arc_sharc_resolve")
+
+    def deposit_paths(self, d_samples: int, n: int, cam, stream: int = 0):
+        self._rc(lib().arc_sharc_deposit_paths(self._h, d_samples, n, (C.c_float * 3)(*cam), stream or None),
+                 "arc_sharc_deposit_paths")
+
+    def query_paths(self, d_samples: int, n: int, cam, d_out: int, stream: int = 0):
+        self._rc(lib().arc_sharc_query_paths(self._h, d_samples, n, (C.c_float * 3)(*cam), d_out, stream or None),
+                 "arc_sharc_query_paths")
 
     def query(self, d_probes: int, n: int, cam, d_out: int, stream: int = 0):
         self._rc(lib().arc_sharc_query(self._h, d_probes, n, (C.c_float * 3)(*cam), d_out, stream or None),

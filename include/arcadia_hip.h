@@ -335,6 +335,13 @@ int arc_sharc_resolve(arc_sharc* s, void* stream);
 int arc_sharc_query(const arc_sharc* s, const arc_sharc_probe* d_queries, uint64_t n, const float cam_pos[3],
                     float* d_out, void* stream);
 
+/* The same deposit / query fed by shaded path samples: the surfel of x1 (normal from its face)
+ * receives `target` (the radiance beyond x1); samples without an x1 hit are skipped (query: 0). */
+int arc_sharc_deposit_paths(arc_sharc* s, const arc_path_sample* d_samples, uint64_t n, const float cam_pos[3],
+                            void* stream);
+int arc_sharc_query_paths(const arc_sharc* s, const arc_path_sample* d_samples, uint64_t n, const float cam_pos[3],
+                          float* d_out, void* stream);
+
 /* Kernel launch shape of the persistent ray-queue kernel; 0 fields = autotuned defaults. */
 typedef struct arc_launch_config {
   uint32_t blocks_per_cu;   /* resident workgroups per compute unit */
