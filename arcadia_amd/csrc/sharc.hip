@@ -277,6 +277,41 @@ __global__ __launch_bounds__(256) void k_sharc_query_paths(const arc_path_sample
   out[i] = lookup(resolved, make_key(ps.pos, nrm, cam, 0.0f, 0.0f, bucket_mask));
 }
 
+// Evicts every surfel whose cell overlaps one of the spheres (x, y, z, r in meters, widened by
+// the 2 cm anchor offset): one workgroup per (sphere, cascade level); each thread walks cells of
+// the sphere's bounding box at that level and probes the 6 face keys of each.
+__global__ __launch_bounds__(256) void k_sharc_invalidate(const float4* __restrict__ spheres, uint32_t bucket_mask,
+                                                          uint32_t* __restrict__ keys, uint4* __restrict__ accum,
+                                                          uint4* __restrict__ resolved) {
+  const float4 sp = spheres[blockIdx.x >> 2];
+  const uint32_t lod = blockIdx.x & 3u;
+  const float s = 0.25f * (float)(1u << lod);
+  const float r = sp.w + 0.05f;
+  const int lo[3] = {(int)floorf((sp.x - r) / s), (int)floorf((sp.y - r) / s), (int)floorf((sp.z - r) / s)};
+  const int hi[3] = {(int)floorf((sp.x + r) / s), (int)floorf((sp.y + r) / s), (int)floorf((sp.z + r) / s)};
+  const uint32_t nx = (uint32_t)(hi[0] - lo[0] + 1), ny = (uint32_t)(hi[1] - lo[1] + 1), nz = (uint32_t)(hi[2] - lo[2] + 1);
+  const uint32_t cells = nx * ny * nz;
+  for (uint32_t c = threadIdx.x; c < cells; c += blockDim.x) {
+    const uint32_t ix = c % nx, iy = (c / nx) % ny, iz = c / (nx * ny);
+    const uint32_t cx = (uint32_t)(lo[0] + (int)ix + 16384), cy = (uint32_t)(lo[1] + (int)iy + 16384),
+                   cz = (uint32_t)(lo[2] + (int)iz + 16384);
+    for (uint32_t nid = 0; nid < 6; ++nid) {
+      const uint32_t bucket =
+          pcg((cx * 73856093u) ^ (cy * 19349663u) ^ (cz * 83492791u) ^ (nid * 2654435761u) ^ (lod * 38069u)) & bucket_mask;
+      const uint32_t key = pcg(bucket ^ (cx * 1013904223u) ^ (cy * 1664525u) ^ (cz * 2891336453u) ^ (nid << 20u) ^
+                               (lod << 24u)) |
+                           kKeyValid;
+      for (uint32_t w = 0; w < kWays; ++w) {
+        const uint32_t slot = bucket * kWays + w;
+        if (atomicCAS(keys + slot, key, 0u) == key) {
+          accum[slot] = make_uint4(0u, 0u, 0u, 0u);
+          resolved[slot] = make_uint4(0u, 0u, 0u, 0u);
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint64_t n) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = 0u;
@@ -290,6 +325,8 @@ struct arc_sharc {
   uint32_t* keys = nullptr;
   uint32_t* accum = nullptr;
   uint4* resolved = nullptr;
+  float* inval = nullptr;  // sphere list of arc_sharc_invalidate
+  uint32_t inval_cap = 0;
 };
 
 extern "C" {
@@ -328,6 +365,7 @@ void arc_sharc_destroy(arc_sharc* s) {
   (void)hipFree(s->keys);
   (void)hipFree(s->accum);
   (void)hipFree(s->resolved);
+  (void)hipFree(s->inval);
   delete s;
 }
 
@@ -387,6 +425,29 @@ int arc_sharc_query(const arc_sharc* s, const arc_sharc_probe* d_queries, uint64
                      (uint32_t)n, cam_pos[0], cam_pos[1], cam_pos[2], s->buckets - 1u, s->resolved,
                      reinterpret_cast<float4*>(d_out));
   return hipGetLastError() == hipSuccess ? ARC_OK : ARC_ERR_HIP;
+}
+
+int arc_sharc_invalidate(arc_sharc* s, const float* spheres, uint32_t n, void* stream) {
+  if (!s || (n && !spheres) || n > (1u << 20)) return ARC_ERR_INVALID;
+  if (n == 0) return ARC_OK;
+  for (uint32_t i = 0; i < n; ++i)
+    if (!(spheres[4 * i + 3] >= 0.0f) || spheres[4 * i + 3] > 64.0f) return ARC_ERR_INVALID;  // <= 64 m
+  if (hipSetDevice(s->device) != hipSuccess) return ARC_ERR_HIP;
+  if (n > s->inval_cap) {
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return ARC_ERR_HIP;
+    (void)hipFree(s->inval);
+    s->inval = nullptr;
+    s->inval_cap = 0;
+    if (hipMalloc(&s->inval, (size_t)n * 16) != hipSuccess) return ARC_ERR_HIP;
+    s->inval_cap = n;
+  }
+  if (hipMemcpyAsync(s->inval, spheres, (size_t)n * 16, hipMemcpyHostToDevice, (hipStream_t)stream) != hipSuccess)
+    return ARC_ERR_HIP;
+  hipLaunchKernelGGL(k_sharc_invalidate, dim3(4 * n), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<const float4*>(s->inval), s->buckets - 1u, s->keys,
+                     reinterpret_cast<uint4*>(s->accum), s->resolved);
+  if (hipGetLastError() != hipSuccess) return ARC_ERR_HIP;
+  return hipStreamSynchronize((hipStream_t)stream) == hipSuccess ? ARC_OK : ARC_ERR_HIP;
 }
 
 int arc_sharc_deposit_paths(arc_sharc* s, const arc_path_sample* d_samples, uint64_t n, const float cam_pos[3],

@@ -217,6 +217,7 @@ _SIGS = [
     ("arc_sharc_read", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("arc_sharc_deposit", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p]),
     ("arc_sharc_resolve", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("arc_sharc_invalidate", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     ("arc_sharc_deposit_paths", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p]),
     ("arc_sharc_query_paths", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_float), C.c_void_p,
                                         C.c_void_p]),
@@ -506,8 +507,9 @@ class HostWorld:
     def clear_patches(self) -> None:
         self._rc(lib().arc_world_host_clear_patches(self._h), "clear_patches")
 
-    def pools(self):
-        """Copies of (grid, chunk_roots, node_pool, brick_pool) after pack()."""
+    def pools(self, copy: bool = True):
+        """(grid, chunk_roots, node_pool, brick_pool) after pack(): copies, or (copy=False) views of
+        the host pools valid until the next edit."""
         g = (C.c_uint32 * 3)()
         r, n, b = C.c_void_p(), C.c_void_p(), C.c_void_p()
         nn, nb = C.c_uint64(), C.c_uint64()
@@ -515,9 +517,11 @@ class HostWorld:
         if rc != ARC_OK:
             raise ArcError(rc, "pools (call pack() first)")
         nch = g[0] * g[1] * g[2]
-        roots = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_uint32)), (nch,)).copy()
-        nodes = np.frombuffer((C.c_char * (nn.value * 16)).from_address(n.value), NODE_DTYPE).copy()
-        bricks = np.frombuffer((C.c_char * (nb.value * 64)).from_address(b.value), BRICK_DTYPE).copy()
+        roots = np.ctypeslib.as_array(C.cast(r, C.POINTER(C.c_uint32)), (nch,))
+        nodes = np.frombuffer((C.c_char * (nn.value * 16)).from_address(n.value), NODE_DTYPE)
+        bricks = np.frombuffer((C.c_char * (nb.value * 64)).from_address(b.value), BRICK_DTYPE)
+        if copy:
+            roots, nodes, bricks = roots.copy(), nodes.copy(), bricks.copy()
         return (g[0], g[1], g[2]), roots, nodes, bricks
 
 
@@ -821,6 +825,12 @@ class Sharc:
 
     def resolve(self, stream: int = 0):
         self._rc(lib().arc_sharc_resolve(self._h, stream or None), "arc_sharc_resolve")
+
+    def invalidate(self, spheres, stream: int = 0):
+        """spheres: (n, 4) x, y, z, radius in meters."""
+        sp = np.ascontiguousarray(np.asarray(spheres, np.float32).reshape(-1, 4))
+        self._rc(lib().arc_sharc_invalidate(self._h, _ptr(sp) if len(sp) else None, len(sp), stream or None),
+                 "arc_sharc_invalidate")
 
     def deposit_paths(self, d_samples: int, n: int, cam, stream: int = 0):
         self._rc(lib().arc_sharc_deposit_paths(self._h, d_samples, n, (C.c_float * 3)(*cam), stream or None),

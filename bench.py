@@ -171,6 +171,168 @@ def batched_leg(args, threads, stream):
     return out
 
 
+def cache_leg(args, threads):
+    """Learned radiance cache vs SHARC (2^22 entries) vs no cache, rel-MSE against the FP64
+    1024-spp path-traced judge at 1920x1080 on the full-frame world with 8 emissive clusters."""
+    import arcadia_amd as A
+    from arcadia_amd import lightcache as LC
+
+    t0 = time.time()
+    world = A.HostWorld((4, 4, 4))
+    world.generate(A.gen_params(A.ARC_GEN_CAVES, seed=args.seed), nthreads=threads)
+    clusters = LC.add_emissive_clusters(world, (4, 4, 4), seed=args.seed, count=8)
+    world.pack()
+    pools = world.pools()
+    del world
+    ctx = A.Context(0)
+    ctx.upload(*pools)
+    cam = A.camera_look((31.3, 58.0, 27.1), 0.7, -0.5, 65.0, 1920, 1080)
+    run = LC.CacheRun(ctx, cam, 1920, 1080, train_size=(341, 192), sharc_log2=20,
+                      nrc=A.NrcParams.default(bbox_size=64.0, log2_table=19), seed=args.seed)
+    print(f"[bench] cache: scene {time.time() - t0:.1f}s, {run.n_eval} eval slots, {run.n_train} training paths "
+          f"per frame", file=sys.stderr, flush=True)
+    ref_s = run.reference(args.cache_ref_spp, progress=lambda k: print(f"[bench] cache: reference {k} spp",
+                                                                        file=sys.stderr, flush=True))
+    for f in range(args.cache_frames):
+        run.train_paths(f)
+        run.train_sharc()
+        run.train_nrc(1)
+    err = run.errors(seed=(1 << 40) + args.seed)
+    ms = run.cache_ms(reps=5)
+    out = {
+        "metric": "rel-MSE vs FP64 1024-spp path-traced judge at 1920x1080 (lower is better)",
+        "nrc": round(err["nrc"], 5),
+        "sharc": round(err["sharc"], 5),
+        "no_cache": round(err["no_cache"], 5),
+        "pt_1spp": round(err["pt_1spp"], 5),
+        "nrc_ms_per_frame": round(ms["nrc_ms"], 4),
+        "sharc_ms_per_frame": round(ms["sharc_ms"], 4),
+        "frames": args.cache_frames,
+        "training_paths_per_frame": int(run.n_train),
+        "ref_spp": run.ref_spp,
+        "ref_seconds": round(ref_s, 2),
+        "scene": "full-frame world + 8 emissive clusters (mats 19-22, r 6..14 voxels), analytic sky, 4-bounce paths",
+        "sharc_entries": 1 << 22,
+        "nrc_model": {"hash_levels": 16, "table": 1 << 19, "mlp": "38->64->64->64->3 FP16 MFMA, FP32 accumulate"},
+    }
+    run.close()
+    ctx.close()
+    return out
+
+
+def destruction_leg(args, threads):
+    """30 frames over the full-frame world; each frame carves >= 1% of the solid voxels with
+    seeded spheres (radius 4..32 voxels), syncs the touched nodes into GPU patches, traces the
+    1080p primary + diffuse frame in one launch, checks a sample of it bit-exact against the
+    oracle on the edited host pools, invalidates SHARC surfels in the carved spheres and takes a
+    neural-cache training step on fresh paths."""
+    import torch
+
+    import arcadia_amd as A
+    from oracle import oracle as O
+
+    world = A.HostWorld((4, 4, 4))
+    world.generate(A.gen_params(A.ARC_GEN_CAVES, seed=args.seed), nthreads=threads)
+    world.pack()
+    ctx = A.Context(0)
+    ctx.upload(*world.pools(copy=False))
+    world.clear_patches()
+    cam = A.camera_look((31.3, 58.0, 27.1), 0.7, -0.5, 65.0, 1920, 1080)
+    p0, pb = A.TraceParams.exact(0), A.TraceParams.exact(2, ignore_water=True)
+    frame = ctx.batch([(cam, 1920, 1080, 250.0)], p0, pb, n_bounces=1, bounce_max_dist_m=26.0, seed=args.seed)
+    tcam = A.camera_look((0.0, 0.0, 0.0), 0.0, 0.0, 60.0, 341, 192)
+    for f in ("eye", "forward", "right", "up"):
+        setattr(tcam, f, getattr(cam, f))
+    tcam.tan_half_fov, tcam.aspect = cam.tan_half_fov, 341 / 192
+    train = ctx.batch([(tcam, 341, 192, 250.0)], p0, pb, n_bounces=4, bounce_max_dist_m=250.0, seed=args.seed)
+    hits = torch.empty(frame.records() * 16, dtype=torch.uint8, device="cuda")
+    thits = torch.empty(train.records() * 16, dtype=torch.uint8, device="cuda")
+    tsamp = torch.empty(train.slots * 96, dtype=torch.uint8, device="cuda")
+    sharc = A.Sharc(20)
+    nrc = A.Nrc(A.NrcParams.default(bbox_size=64.0, log2_table=19))
+    shade = A.ShadeParams.default()
+    rays0 = A.view_rays(cam, 1920, 1080, 250.0)
+    n = frame.slots
+    rng = np.random.default_rng(args.seed + 5)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    t_edit, t_patch, t_trace, t_cache, carved, exact, patched_mb = [], [], [], [], [], 0, []
+    for f in range(args.destruction_frames):
+        solid = sum(world.chunk_info(i)["solid"] for i in range(64))
+        quota = solid // 100
+        t0 = time.perf_counter()
+        done, spheres = 0, []
+        while done < quota:
+            c = rng.uniform(0.0, 64.0, 3)
+            r = rng.uniform(4.0, 32.0) / 16.0
+            done += world.carve_crater(tuple(c), r)
+            spheres.append((c[0], c[1], c[2], r))
+        t1 = time.perf_counter()
+        pl = world.patch_list() if f == 0 else None
+        if pl is not None:
+            patched_mb.append(sum(p["nodes"].nbytes + p["bricks"].nbytes for p in pl) / 2**20)
+        ctx.patch_world(world)
+        t2 = time.perf_counter()
+        frame.set_seed(args.seed * 7919 + f)
+        ev[0].record()
+        frame.trace(hits.data_ptr())
+        ev[1].record()
+        sharc.invalidate(np.array(spheres, np.float32))
+        train.set_seed(args.seed * 104729 + f)
+        train.trace(thits.data_ptr())
+        train.shade(thits.data_ptr(), shade, d_samples=tsamp.data_ptr())
+        sharc.deposit_paths(tsamp.data_ptr(), train.slots, tuple(cam.eye))
+        sharc.resolve()
+        nrc.train(tsamp.data_ptr(), train.slots)
+        ev[2].record()
+        torch.cuda.synchronize()
+        t_edit.append(t1 - t0)
+        t_patch.append(t2 - t1)
+        t_trace.append(ev[0].elapsed_time(ev[1]))
+        t_cache.append(ev[1].elapsed_time(ev[2]))
+        carved.append(done)
+        # bit-exact check of a sample of this frame against the oracle on the edited host pools
+        got = hits.cpu().numpy().view(A.HIT_DTYPE).reshape(2, n)
+        sample = np.sort(rng.choice(n, 4096, replace=False))
+        pools = world.pools(copy=False)
+        ow = O.World(*pools)
+        full, _ = O.trace_world(ow, p0, rays0[sample], threads)
+        h0 = O.compact(full)
+        ok = len(np.nonzero((got[0][sample].view(np.uint32).reshape(-1, 4) != h0.view(np.uint32).reshape(-1, 4)).any(1))[0]) == 0
+        br = np.zeros(len(sample), A.RAY_DTYPE)
+        for k, s_ in enumerate(sample):
+            br[k] = A.bounce_rays_dense(rays0[s_:s_ + 1], h0[k:k + 1], index_base=int(s_),
+                                        seed=args.seed * 7919 + f, max_dist_m=26.0)[0]
+        full1, _ = O.trace_world(ow, pb, br, threads)
+        h1 = O.compact(full1)
+        ok = ok and len(np.nonzero((got[1][sample].view(np.uint32).reshape(-1, 4) !=
+                                    h1.view(np.uint32).reshape(-1, 4)).any(1))[0]) == 0
+        exact += int(ok)
+        print(f"[bench] destruction frame {f}: carved {done} ({len(spheres)} spheres) edit {t1 - t0:.3f}s "
+              f"patch {t2 - t1:.3f}s trace {t_trace[-1]:.3f}ms cache {t_cache[-1]:.3f}ms exact {ok}",
+              file=sys.stderr, flush=True)
+    t = hits.view(torch.float32)[0::4].reshape(2, n)
+    rays = int(((t != -16.0).sum()).item())
+    out = {
+        "metric": "destruction sequence: per-frame edit + patch + 1080p primary/diffuse traversal + cache update",
+        "frames": args.destruction_frames,
+        "frames_bit_exact": exact,
+        "carved_voxels_per_frame": int(np.mean(carved)),
+        "host_edit_ms": round(1e3 * float(np.mean(t_edit)), 2),
+        "patch_upload_ms": round(1e3 * float(np.mean(t_patch)), 2),
+        "first_frame_patch_mib": round(patched_mb[0], 1) if patched_mb else None,
+        "trace_ms": round(float(np.mean(t_trace)), 4),
+        "trace_grays": round(rays / (float(np.mean(t_trace)) * 1e-3) / 1e9, 4),
+        "cache_ms": round(float(np.mean(t_cache)), 4),
+        "check": "4096 random pixels per frame: primary and bounce records vs the oracle on the edited host pools",
+    }
+    sharc.close()
+    nrc.close()
+    frame.close()
+    train.close()
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +344,11 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shapes")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
+    ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache workload")
+    ap.add_argument("--cache-ref-spp", type=int, default=1024)
+    ap.add_argument("--cache-frames", type=int, default=60)
+    ap.add_argument("--no-destruction", action="store_true", help="skip the destruction sequence")
+    ap.add_argument("--destruction-frames", type=int, default=30)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device copy of this many MiB (scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -192,6 +359,8 @@ def main() -> int:
     if args.profile:
         args.steps, args.warmup, args.no_cpu_baseline, args.no_autotune = min(args.steps, 5), 1, True, True
         args.no_batched = True
+        args.no_cache = True
+        args.no_destruction = True
 
     import torch
 
@@ -314,6 +483,13 @@ def main() -> int:
         out["batched"] = batched_leg(args, threads, stream)
         print(f"[bench] batched: {out['batched']['value']} Grays/s, {out['batched']['ms_per_frame']} ms/frame",
               file=sys.stderr, flush=True)
+    if not args.no_cache:
+        out["cache"] = cache_leg(args, threads)
+        c = out["cache"]
+        print(f"[bench] cache: rel-MSE nrc {c['nrc']} sharc {c['sharc']} none {c['no_cache']}; "
+              f"ms nrc {c['nrc_ms_per_frame']} sharc {c['sharc_ms_per_frame']}", file=sys.stderr, flush=True)
+    if not args.no_destruction:
+        out["destruction"] = destruction_leg(args, threads)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -335,6 +335,10 @@ int arc_sharc_resolve(arc_sharc* s, void* stream);
 int arc_sharc_query(const arc_sharc* s, const arc_sharc_probe* d_queries, uint64_t n, const float cam_pos[3],
                     float* d_out, void* stream);
 
+/* Evict every surfel (all cascade levels and faces) whose cell overlaps one of n spheres
+ * {x, y, z, radius} in meters (radius <= 64 m), e.g. the carve spheres of an edit, so no stale
+ * radiance is served where the geometry changed. Blocking (the sphere list is copied). */
+int arc_sharc_invalidate(arc_sharc* s, const float* spheres, uint32_t n, void* stream);
 /* The same deposit / query fed by shaded path samples: the surfel of x1 (normal from its face)
  * receives `target` (the radiance beyond x1); samples without an x1 hit are skipped (query: 0). */
 int arc_sharc_deposit_paths(arc_sharc* s, const arc_path_sample* d_samples, uint64_t n, const float cam_pos[3],

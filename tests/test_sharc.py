@@ -244,3 +244,34 @@ def test_gpu_sharc_matches_oracle_per_key(ctx):
         assert ok.sum() > 0.5 * len(ok)
         assert np.array_equal(got[ok].view(np.uint32), want[ok].view(np.uint32))
     sh.close()
+
+
+@pytest.mark.gpu
+def test_gpu_invalidate_evicts_surfels_inside_spheres(ctx):
+    import torch
+
+    sh = A.Sharc(14)
+    rng = np.random.default_rng(3)
+    cam = (30.0, 12.0, 30.0)
+    s = random_deposits(rng, 4000, cam)
+    d = torch.from_numpy(s.view(np.uint8).copy()).cuda()
+    sh.deposit(d.data_ptr(), len(s), cam)
+    sh.resolve()
+    spheres = np.array([[30.0, 12.0, 30.0, 6.0], [40.0, 20.0, 22.0, 3.0]], np.float32)
+    probes = np.zeros(len(s), O.SHARC_PROBE_DTYPE)
+    probes["pos"], probes["normal"] = s["pos"], s["normal"]
+    dp = torch.from_numpy(probes.view(np.uint8).copy()).cuda()
+    out = torch.zeros(len(s) * 4, dtype=torch.float32, device="cuda")
+    sh.query(dp.data_ptr(), len(s), cam, out.data_ptr())
+    before = out.cpu().numpy().reshape(-1, 4)[:, 3]
+    sh.invalidate(spheres)
+    sh.query(dp.data_ptr(), len(s), cam, out.data_ptr())
+    after = out.cpu().numpy().reshape(-1, 4)[:, 3]
+    p = s["pos"].astype(np.float64)
+    dist = np.min(np.linalg.norm(p[:, None, :] - spheres[None, :, :3], axis=2) - spheres[None, :, 3], axis=1)
+    cheb = np.min(np.abs(p[:, None, :] - spheres[None, :, :3]).max(axis=2) - spheres[None, :, 3], axis=1)
+    inside, far = dist < -0.05, cheb > 2.0 + 0.1  # far: outside each sphere's box widened by the largest cell
+    assert before[inside].all() and inside.sum() > 30
+    assert not after[inside].any()
+    assert np.array_equal(after[far], before[far]) and far.sum() > 1000
+    sh.close()
