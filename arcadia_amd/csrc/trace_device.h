@@ -9,10 +9,11 @@
 // What differs from the WGSL text and why it cannot change a result:
 //   * the 2-slot chunk cache keys on the flat chunk index instead of the chunk coordinate
 //     (equal indices <=> equal coordinates inside the grid; both start at an impossible key);
-//   * the ancestor restart keeps the same three cached Node64s in registers — a restart is
-//     equivalent to a root descent because the cached node at shift s always covers the
-//     current voxel's bits >= s+2 (the XOR `diff` tests), so results are identical and only
-//     the number of node loads differs.
+//   * the ancestor restart keeps the cached Node64s (chunk root, level-1 and level-2 ancestors in
+//     LDS, the deepest node in registers) and restarts at the level the reference's restart
+//     conditions select; a restart is equivalent to a root descent because the cached node at
+//     shift s always covers the current voxel's bits >= s+2 (the XOR `diff` tests), so results
+//     are identical and only the number of node loads differs.
 #pragma once
 
 #include <hip/hip_runtime.h>
