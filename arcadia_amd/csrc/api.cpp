@@ -30,7 +30,8 @@ struct arc_ctx {
   arc_brick64* d_bricks = nullptr;
   uint64_t n_nodes = 0, node_cap = 0, n_bricks = 0, brick_cap = 0;
   bool has_world = false;
-  // persistent-queue counters (2 x u32), one pair per concurrently usable stream slot
+  // persistent-queue counters: words [0, kQueueWords) for the traversal kernel, [16, 16 +
+  // kQueueWords) for the path kernel
   uint32_t* d_queue = nullptr;
   // scratch for the host-pointer entry point
   arc_ray* d_rays = nullptr;
@@ -71,6 +72,8 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
   } while (0)
 
 // blocks_per_cu 0 = as many workgroups as the kernel's occupancy keeps resident.
+static_assert(16u + arcdev::kQueueWords <= 64u, "queue words");
+
 arc_launch_config default_config() {
   arc_launch_config c;
   c.blocks_per_cu = 0;
@@ -600,7 +603,7 @@ int batch_launch(arc_batch* b, arc_hit* d_hits, hipStream_t s) {
   a.bounce_max_dist_m = b->bounce_tmax;
   a.seed = b->seed;
   a.out = d_hits;
-  a.queue = c->d_queue + 8;  // the path kernel's own counter pair
+  a.queue = c->d_queue + 16;  // the path kernel's own counters
   uint32_t bpc = b->cfg.blocks_per_cu;
   if (bpc == 0) {
     hipError_t oe = arcdev::path_occupancy(b->cfg.block_threads, &bpc);

@@ -7,8 +7,8 @@
 // division), so hit voxel, face, t and step count equal the reference's source semantics.
 //
 // What differs from the WGSL text and why it cannot change a result:
-//   * the 2-slot chunk cache keys on the flat chunk index instead of the chunk coordinate
-//     (equal indices <=> equal coordinates inside the grid; both start at an impossible key);
+//   * the reference's 2-slot chunk cache only saves chunk_roots reads: every chunk change reads
+//     chunk_roots[key] directly, which yields the same root index;
 //   * the ancestor restart keeps the cached Node64s (chunk root, level-1 and level-2 ancestors in
 //     LDS, the deepest node in registers) and restarts at the level the reference's restart
 //     conditions select; a restart is equivalent to a root descent because the cached node at
@@ -26,6 +26,8 @@ namespace arcdev {
 struct DevWorld {
   uint32_t gx, gy, gz;
   uint32_t pad;
+  float ext[3];     // world extent in voxels, (float)g * 256 (binary32, computed on the host)
+  float ext_in[3];  // ext - 1e-3 (binary32): the clamp bound of the start voxel
   const uint32_t* __restrict__ roots;
   const uint4* __restrict__ nodes;
   const uint32_t* __restrict__ bricks;  // 16 words per brick

@@ -10,7 +10,7 @@
 //   * finished lanes are not resolved in place: the hit record (cell entry planes, tie-break
 //     normal, hash, AO) is computed once per refill for all finished lanes together, not in
 //     every wave trip in which some lane of 64 hits;
-//   * cold per-lane state (cached ancestors, chunk root node, 2-slot chunk cache) lives in LDS
+//   * cold per-lane state (cached ancestors, chunk root node) lives in LDS
 //     indexed by level: a descent writes one slot, a restart reads one slot;
 //   * the lane keeps `diff` (XOR of the last step's voxels) instead of prev_vi, and keeps the
 //     node of its deepest level, which is the restart node whenever the restart level equals
@@ -19,13 +19,13 @@
 // output: the cached-ancestor restart is equivalent to a root descent (the cached node at shift
 // s covers the current voxel's bits >= s+2 whenever the reference uses it).
 //
-// Queue: a wave claims windows of `batch` consecutive rays with one atomic and refills its
-// idle lanes from the window. The queue can be split into kSegments contiguous segments (each
-// workgroup starting on segment blockIdx % kSegments and moving on when it is exhausted); one
-// segment measured fastest on MI355X: with all waves sweeping the ray array together, the
-// concurrently traced rays form one coherent band of screen tiles instead of eight. The last
-// workgroup to finish resets the counters, which keeps a launch self-contained (no memset
-// node: graph-safe).
+// Queue: a wave claims windows of `batch` consecutive rays with one atomic on a single counter
+// and refills its idle lanes from the window, so all waves of all XCDs sweep the ray array
+// together and the rays in flight form one coherent band of screen tiles. Measured on MI355X:
+// per-XCD counters (the array cut into stripes of 1..4096 windows dealt round-robin to the XCDs,
+// each XCD draining its own stripes first) ran 15-25% slower at every stripe size, and 8
+// contiguous per-XCD segments 65% slower. The last workgroup to finish resets the counters,
+// which keeps a launch self-contained (no memset node: graph-safe).
 #include <hip/hip_runtime.h>
 
 #include "raymath.h"
@@ -34,7 +34,6 @@
 
 namespace arcdev {
 
-constexpr uint32_t kSegments = 1u;
 
 // packed per-lane flags
 enum : uint32_t {
@@ -44,6 +43,7 @@ enum : uint32_t {
   F_HIT = 7,     // 1 bit: finished with a hit
   F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
   F_DEPTH = 9,   // 3 bits: path kernel: 0 = primary ray, d = d-th bounce
+  F_DESC = 12,   // 1 bit: descent parked at the pass cap, continues at the F_CSHIFT level
 };
 
 __device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
@@ -70,11 +70,9 @@ struct Lane {
 // Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
 //   slot 0: chunk root node (after a hit: the node the hit resolved in),
 //   slot 1: cached level-1 ancestor, slot 2: cached level-2 ancestor (slot of shift s:
-//   (6 - s) / 2; the level-3 ancestor is the lane's `node`),
-//   slot 3: 2-slot chunk cache {k0, k1, r0, r1} (flat chunk index -> root node index).
+//   (6 - s) / 2; the level-3 ancestor is the lane's `node`).
 constexpr uint32_t kMaxBlock = 256u;
-constexpr uint32_t kMtfSlot = 3u;
-__device__ __forceinline__ uint32_t slot_of(uint32_t sh) { return (6u - sh) >> 1; }
+constexpr uint32_t kColdSlots = 3u;
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -116,81 +114,151 @@ __device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const D
   return L.step >= max_steps || L.t >= L.tl || L.vx >= wx || L.vy >= wy || L.vz >= wz;
 }
 
+// Restart level of an iteration without a chunk change (traversal64.wgsl:374-386). The
+// reference tests c3 (diff < 4, min_shift == 0, cached_shift == 0), then c2 (diff < 16, both
+// <= 2), then c1 (diff < 64, both <= 4), else restarts at the chunk root. With m =
+// max(min_shift, cached_shift), both even, and dl the lowest level whose block holds both the old
+// and the new voxel (0 for diff < 4, 2 below 16, 4 below 64, else 6), the first condition that
+// holds is the one of level max(dl, m): that is the restart level. dl = bit length of diff,
+// minus one, rounded down to even, capped at 6.
+__device__ __forceinline__ uint32_t restart_shift(uint32_t diff, uint32_t minsh, uint32_t cs) {
+  const uint32_t dl = umin_(6u, (31u - (uint32_t)__builtin_clz(diff | 1u)) & ~1u);
+  return umax_(dl, umax_(minsh, cs));
+}
+
+// Address of the LDS slot of the cached node at shift `sh` (slot (6 - sh) / 2).
+__device__ __forceinline__ uint4* cold_slot(uint4* C, uint32_t sh) {
+  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (kMaxBlock * 8u));
+}
+
+// Chunk roots staged in LDS for worlds of at most kLdsChunks chunks: the root index and the root
+// node of every chunk, read at each chunk change in place of chunk_roots[key] followed by
+// nodes[root] (two dependent global loads on the longest dependency chain of a wave trip).
+constexpr uint32_t kLdsChunks = 256u;
+struct RootLds {
+  uint4 node[kLdsChunks];
+  uint32_t idx[kLdsChunks];
+};
+
+// Fills `R` from the pools; returns false (block-uniform) when the world has more chunks.
+__device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLds& R) {
+  const uint32_t nc = w.gx * w.gy * w.gz;
+  if (nc > kLdsChunks) return false;
+  for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+    const uint32_t r = w.roots[c];
+    R.idx[c] = r;
+    R.node[c] = r == ARC_EMPTY_CHUNK_SENTINEL ? make_uint4(0u, 0u, 0u, 0u) : w.nodes[r];
+  }
+  __syncthreads();
+  return true;
+}
+
+// Descent passes (child-bit tests) one wave trip may run for a lane before it parks the descent
+// and continues it on the next trip; 4 = never park (a descent has at most four levels).
+constexpr uint32_t kMaxPasses = 4u;
+
 // One outer iteration of a live lane whose guards passed (traversal64.wgsl:311-552).
-template <uint32_t SEM>
-__device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C) {
-  const uint32_t sidx = L.step;
-  L.step = sidx + 1u;
-  const uint32_t diff = L.diff;
+// Instruction count is what bounds the kernel, so the sequence is written for few VALU ops:
+//   * the child test and the child slot come from one 64-bit shift of the child mask that puts
+//     bit b at the top: the sign gives has_child(b), the popcount minus one gives
+//     popcount(mask & ((1 << b) - 1));
+//   * the descent loop only tests and loads; the material of a hit candidate (a set bit in a
+//     leaf or at the LOD floor) is resolved after the loop;
+//   * cached_shift always equals the level the iteration ended on, so one field carries both;
+//   * the chunk root is read from the chunk_roots array on every chunk change (the array is a
+//     few KB and stays in cache), which gives the same root as the reference's 2-slot cache;
+//   * the box step computes the clamped inside coordinate of all three axes and selects, with no
+//     branches;
+//   * WATER = false compiles the ignore_water tests out (ray classes that never set it).
+template <uint32_t SEM, bool WATER>
+__device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C, const RootLds& R,
+                                          bool lds_roots) {
   uint32_t st = L.st;
-  uint32_t cs = fget(st, F_CSHIFT, 3);
-  uint32_t span = 256u;  // box to step over when the iteration ends without a hit
-  bool descend = true;
   uint4 node = L.node;
-  uint32_t sh = 6u;
-  uint32_t minsh = 0u;
-  if (SEM == ARC_SEM_WORLD && diff >= 256u) {  // chunk change: 2-slot cache (:316-335)
-    const uint32_t key = (L.vx >> 8) + (L.vy >> 8) * w.gx + (L.vz >> 8) * w.gx * w.gy;
-    cs = 6u;
-    const uint4 m = C[kMtfSlot * kMaxBlock];  // {k0, k1, r0, r1}
-    const bool slot1 = key == m.y;
-    const uint32_t root = slot1 ? m.w : w.roots[key];
-    C[kMtfSlot * kMaxBlock] = make_uint4(key, m.x, root, m.z);
-    if (root == ARC_EMPTY_CHUNK_SENTINEL) {
-      descend = false;  // step over the empty chunk (:337-352)
-    } else {
-      node = w.nodes[root];
-      C[0] = node;
+  uint32_t sh = fget(st, F_CSHIFT, 3);
+  uint32_t span_mul = 1u;  // the box stepped over is span_mul << sh
+  bool descend = true;
+  uint32_t minsh;
+  if (kMaxPasses < 4u && fget(st, F_DESC, 1)) {  // continue a parked descent
+    minsh = lod_min_shift<SEM>(p, L.t, L.step - 1u);
+  } else {
+    const uint32_t sidx = L.step;
+    L.step = sidx + 1u;
+    minsh = lod_min_shift<SEM>(p, L.t, sidx);
+    const uint32_t diff = L.diff;
+    if (SEM == ARC_SEM_WORLD && diff >= 256u) {  // chunk change (:316-335)
+      const uint32_t key = (L.vx >> 8) + __umul24(L.vy >> 8, w.gx) + __umul24(L.vz >> 8, w.gx * w.gy);
+      uint32_t root;
+      if (lds_roots) {
+        root = R.idx[key];
+        node = R.node[key];
+      } else {
+        root = w.roots[key];
+        if (root != ARC_EMPTY_CHUNK_SENTINEL) node = w.nodes[root];
+      }
+      sh = 6u;
+      if (root == ARC_EMPTY_CHUNK_SENTINEL) {
+        descend = false;  // step over the empty chunk (:337-352)
+        span_mul = 4u;
+      } else {
+        C[0] = node;
+      }
+    } else {  // ancestor restart (:374-386)
+      const uint32_t rs = restart_shift(diff, minsh, sh);
+      if (rs != sh) node = *cold_slot(C, rs);
+      sh = rs;
     }
   }
+  bool cand = false;  // set bit at a leaf or at the LOD floor: hit unless the voxel is air/water
   if (descend) {
-    minsh = lod_min_shift<SEM>(p, L.t, sidx);
-    if (!(SEM == ARC_SEM_WORLD && diff >= 256u)) {  // ancestor restart (:374-386)
-      const bool c3 = diff < 4u && minsh == 0u && cs == 0u;
-      const bool c2 = !c3 && diff < 16u && minsh <= 2u && cs <= 2u;
-      const bool c1 = !c3 && !c2 && diff < 64u && minsh <= 4u && cs <= 4u;
-      sh = c3 ? 0u : (c2 ? 2u : (c1 ? 4u : 6u));
-      if (sh != cs) node = C[slot_of(sh) * kMaxBlock];
-      cs = sh;
-    }
-    const uint32_t lx = L.vx & 255u, ly = L.vy & 255u, lz = L.vz & 255u;
+    uint32_t loads = 0u;
     for (;;) {  // descent (:390-552), at most four levels
-      const uint32_t b = (((lz >> sh) & 3u) << 4) | (((ly >> sh) & 3u) << 2) | ((lx >> sh) & 3u);
-      if (SEM == ARC_SEM_WORLD && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
-          ((node.w >> 24) & 0xFFu) == ARC_MAT_WATER) {
-        span = 4u << sh;
+      const uint32_t b = (__builtin_amdgcn_ubfe(L.vz, sh, 2) << 4) | (__builtin_amdgcn_ubfe(L.vy, sh, 2) << 2) |
+                         __builtin_amdgcn_ubfe(L.vx, sh, 2);
+      if (SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
+          (node.w >> 24) == ARC_MAT_WATER) {
+        span_mul = 4u;
         break;
       }
-      span = 1u << sh;
-      if (!has_child(node.x, node.y, b)) break;
-      // Level-3 nodes are leaves in every pool the upload check accepts; `sh == 0` only bounds
-      // the descent if a malformed node slipped in through a patch.
-      const bool is_leaf = (node.z & ARC_IS_LEAF_FLAG) != 0u;
-      if (is_leaf || sh == 0u || sh <= minsh) {
-        uint32_t mat = (node.w >> 24) & 0xFFu;
-        if (is_leaf && (node.z & ARC_IS_UNIFORM_FLAG) == 0u) {
-          const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
-          mat = (word >> ((b & 3u) * 8u)) & 0xFFu;
-        }
-        if (SEM == ARC_SEM_WORLD && p.ignore_water && mat == ARC_MAT_WATER) break;
-        if (mat != ARC_MAT_AIR) {  // hit: resolved at finalize (:467-523)
-          L.mat = mat;
-          L.node = node;
-          C[0] = node;
-          L.st = fput(fput(fput(fput(st, F_CSHIFT, 3, cs), F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
-          return;
-        }
+      const uint64_t below = (((uint64_t)node.y << 32) | node.x) << (63u - b);
+      if ((int64_t)below >= 0) break;  // no child at b
+      // Level-3 nodes are leaves in every pool the upload check accepts; sh == 0 (implied by
+      // sh <= minsh) only bounds the descent if a malformed node slipped in through a patch.
+      if ((node.z & ARC_IS_LEAF_FLAG) != 0u || sh <= minsh) {
+        cand = true;
         break;
       }
-      node = w.nodes[child_offset(node, b)];
+      node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
       sh -= 2u;
-      cs = sh;
-      if (sh != 0u) C[slot_of(sh) * kMaxBlock] = node;
+      if (sh != 0u) *cold_slot(C, sh) = node;
+      if (kMaxPasses < 4u && ++loads == kMaxPasses) {  // park: the next trip tests `node`
+        L.node = node;
+        L.st = fput(fput(st, F_CSHIFT, 3, sh), F_DESC, 1, 1u);
+        return;
+      }
+    }
+  }
+  if (cand) {  // :441-538
+    const uint32_t b = (__builtin_amdgcn_ubfe(L.vz, sh, 2) << 4) | (__builtin_amdgcn_ubfe(L.vy, sh, 2) << 2) |
+                       __builtin_amdgcn_ubfe(L.vx, sh, 2);
+    uint32_t mat = node.w >> 24;
+    if ((node.z & (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG)) == ARC_IS_LEAF_FLAG) {
+      const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
+      mat = __builtin_amdgcn_ubfe(word, (b & 3u) * 8u, 8);
+    }
+    const bool skip_water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && mat == ARC_MAT_WATER;
+    if (!skip_water && mat != ARC_MAT_AIR) {  // hit: resolved at finalize (:467-523)
+      L.mat = mat;
+      L.node = node;
+      L.st = fput(fput(fput(fput(fput(st, F_CSHIFT, 3, sh), F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u),
+                  F_DESC, 1, 0u);
+      return;
     }
   }
   L.node = node;
   // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292)
-  const uint32_t mk = ~(span - 1u);
+  const uint32_t span = span_mul << sh;
+  const uint32_t mk = 0u - span;
   const uint32_t bx = L.vx & mk, by = L.vy & mk, bz = L.vz & mk;
   const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
   const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
@@ -199,18 +267,22 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   const float next_t = fmaxf(L.t, t_exit);
   const bool is_x = tx <= ty && tx <= tz;
   const bool is_y = !is_x && (ty <= tz);
-  const bool is_z = !is_x && !is_y;
   const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
   const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
-  const uint32_t nvx = is_x ? ex - (px ? 0u : 1u) : umin_(umax_(cvt_u32(ipx), bx), bx + span - 1u);
-  const uint32_t nvy = is_y ? ey - (py ? 0u : 1u) : umin_(umax_(cvt_u32(ipy), by), by + span - 1u);
-  const uint32_t nvz = is_z ? ez - (pz ? 0u : 1u) : umin_(umax_(cvt_u32(ipz), bz), bz + span - 1u);
+  const uint32_t last = span - 1u;
+  const uint32_t cx = umin_(umax_(cvt_u32(ipx), bx), bx + last);
+  const uint32_t cy = umin_(umax_(cvt_u32(ipy), by), by + last);
+  const uint32_t cz = umin_(umax_(cvt_u32(ipz), bz), bz + last);
+  const uint32_t fx = px ? ex : bx - 1u, fy = py ? ey : by - 1u, fz = pz ? ez : bz - 1u;
+  const uint32_t nvx = is_x ? fx : cx;
+  const uint32_t nvy = is_y ? fy : cy;
+  const uint32_t nvz = (is_x || is_y) ? cz : fz;
   L.diff = (nvx ^ L.vx) | (nvy ^ L.vy) | (nvz ^ L.vz);
   L.vx = nvx;
   L.vy = nvy;
   L.vz = nvz;
   L.t = next_t;
-  st = fput(st, F_CSHIFT, 3, cs);
+  st = fput(fput(st, F_CSHIFT, 3, sh), F_DESC, 1, 0u);
   if (iteration_guards_fail<SEM>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
   L.st = st;
 }
@@ -237,7 +309,7 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
     L.ix = 1.0f / sx;
     L.iy = 1.0f / sy;
     L.iz = 1.0f / sz;
-    const float wx = (float)w.gx * 256.0f, wy = (float)w.gy * 256.0f, wz = (float)w.gz * 256.0f;
+    const float wx = w.ext[0], wy = w.ext[1], wz = w.ext[2];
     // intersect_aabb (traversal64.wgsl:134-142)
     const float tnx = ((L.ix > 0.0f ? 0.0f : wx) - L.ox) * L.ix;
     const float tny = ((L.iy > 0.0f ? 0.0f : wy) - L.oy) * L.iy;
@@ -253,10 +325,9 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
     L.tl = fminf(wh_y, max_t_v);
     const float tt = L.t + 1e-4f;
     const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
-    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, wx - 1e-3f));
-    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, wy - 1e-3f));
-    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, wz - 1e-3f));
-    C[kMtfSlot * kMaxBlock] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, ARC_EMPTY_CHUNK_SENTINEL, ARC_EMPTY_CHUNK_SENTINEL);
+    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, w.ext_in[0]));
+    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, w.ext_in[1]));
+    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, w.ext_in[2]));
   } else {
     const uint32_t root = w.roots[0];
     if (root == ARC_EMPTY_CHUNK_SENTINEL) return;
@@ -382,7 +453,7 @@ __device__ __forceinline__ void store_full(arc_trace_result* out, const arc_ray*
   if (hit) {
     const HitInfo h = resolve_hit<SEM>(L, r.dx, r.dy, r.dz);
     const uint32_t sh = fget(L.st, F_SHIFT, 3);
-    const uint4 node = C[0];  // the node the hit resolved in
+    const uint4 node = L.node;  // the node the hit resolved in
     t_vox = h.exact_t;
     n0 = h.axis == 0 ? h.nrm_axis : 0.0f;
     n1 = h.axis == 1 ? h.nrm_axis : 0.0f;
@@ -461,27 +532,39 @@ __device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const ar
   else store_compact<SEM>(reinterpret_cast<arc_hit*>(out), rays, L);
 }
 
+// The last workgroup out resets the counters for the next launch on this stream.
+__device__ __forceinline__ void queue_finish(uint32_t* queue, bool leader) {
+  __syncthreads();
+  if (leader) {
+    __threadfence();
+    const uint32_t done = atomicAdd(queue + 1, 1u);
+    if (done == gridDim.x - 1u) {
+      for (uint32_t s = 0; s < kQueueWords; ++s) atomicExch(queue + s, 0u);
+      __threadfence();
+    }
+  }
+}
+
 // Persistent ray-queue kernel.
-// queue[0..kSegments-1]: rays claimed from each segment; queue[kSegments]: finished workgroups.
+// queue[0]: rays claimed; queue[1]: finished workgroups (both zero between launches).
 // Occupancy target: the compact variants fit 64 VGPRs (8 waves/SIMD) without spilling; the
 // full-record variants carry more live values through finalize and keep the default budget.
-template <uint32_t SEM, bool FULL>
+template <uint32_t SEM, bool FULL, bool WATER>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 8, 8))) void k_trace(
     DevWorld w, DevParams p, const arc_ray* __restrict__ rays,
                                                typename OutPtr<FULL>::T* __restrict__ out, uint32_t n,
                                                uint32_t* __restrict__ queue, uint32_t refill, uint32_t batch) {
-  __shared__ uint4 s_cold[4 * kMaxBlock];
+  __shared__ uint4 s_cold[kColdSlots * kMaxBlock];
+  __shared__ RootLds s_roots;
   uint4* const C = s_cold + threadIdx.x;
+  const bool lds_roots = SEM == ARC_SEM_WORLD && stage_roots(w, s_roots);
   const uint32_t lane = __lane_id();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   Lane L;
   L.st = 0u;
   L.step = 0u;
   L.idx = 0u;
-  // wave-uniform queue state
-  uint32_t seg = blockIdx.x % kSegments;
-  uint32_t exhausted = 0u;  // segments found exhausted so far
-  uint32_t wb = 0u, we = 0u;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+  uint32_t wb = 0u, we = 0u;  // wave-uniform: window of rays being handed out
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
@@ -501,21 +584,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
       while (assigned < nidle) {
         if (wb >= we) {
           uint32_t got = 0u;
-          if (lane == 0) got = atomicAdd(queue + seg, batch);
+          if (lane == 0) got = atomicAdd(queue, batch);
           got = rfl(__shfl(got, 0));
-          const uint32_t s_begin = (uint32_t)(((uint64_t)n * seg) / kSegments);
-          const uint32_t s_end = (uint32_t)(((uint64_t)n * (seg + 1u)) / kSegments);
-          const uint32_t lo = s_begin + umin_(got, s_end - s_begin);
-          if (lo >= s_end) {
-            seg = (seg + 1u) % kSegments;
-            if (++exhausted >= kSegments) {
-              drained = true;
-              break;
-            }
-            continue;
+          const uint32_t lo = umin_(got, n);
+          if (lo >= n) {
+            drained = true;
+            break;
           }
           wb = lo;
-          we = umin_(lo + batch, s_end);
+          we = umin_(lo + batch, n);
         }
         const uint32_t take = umin_(nidle - assigned, we - wb);
         if (idle_lane && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);
@@ -529,19 +606,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
       }
       continue;  // re-ballot: freshly initialised lanes may already be done
     }
-    if (!idle_lane) iteration<SEM>(w, p, L, C);
+    if (!idle_lane) iteration<SEM, WATER>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C);
-  // Last workgroup out resets the queue for the next launch on this stream.
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t done = atomicAdd(queue + kSegments, 1u);
-    if (done == gridDim.x - 1u) {
-      for (uint32_t s = 0; s <= kSegments; ++s) atomicExch(queue + s, 0u);
-      __threadfence();
-    }
-  }
+  queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
 
 // ---------------------------------------------------------------------------------------
@@ -652,18 +720,20 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
   }
 }
 
-// queue[0]: slots claimed, queue[1]: finished workgroups (both zero between launches).
+// queue[0]: slots claimed; queue[1]: finished workgroups (both zero between launches).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
     DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
     uint32_t refill, uint32_t batch) {
-  __shared__ uint4 s_cold[4 * kMaxBlock];
+  __shared__ uint4 s_cold[kColdSlots * kMaxBlock];
+  __shared__ RootLds s_roots;
   uint4* const C = s_cold + threadIdx.x;
+  const bool lds_roots = stage_roots(w, s_roots);
   const uint32_t lane = __lane_id();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   Lane L;
   L.st = 0u;
   L.step = 0u;
   L.idx = 0u;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   float tmax = 0.0f;
   uint32_t wb = 0u, we = 0u;
   bool drained = false;
@@ -701,18 +771,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       continue;
     }
     if (drained && nidle == 64u) break;
-    if (!idle_lane) iteration<ARC_SEM_WORLD>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C);
+    if (!idle_lane)
+      iteration<ARC_SEM_WORLD, true>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots, lds_roots);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t done = atomicAdd(queue + 1, 1u);
-    if (done == gridDim.x - 1u) {
-      atomicExch(queue, 0u);
-      atomicExch(queue + 1, 0u);
-      __threadfence();
-    }
-  }
+  queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
 
 }  // namespace arcdev
@@ -721,6 +783,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // Launchers
 // ---------------------------------------------------------------------------------------
 namespace arcdev {
+
+static DevWorld make_world(const uint32_t grid[3], const uint32_t* roots, const arc_node64* nodes,
+                           const arc_brick64* bricks) {
+  DevWorld w;
+  w.gx = grid[0];
+  w.gy = grid[1];
+  w.gz = grid[2];
+  w.pad = 0;
+  for (int a = 0; a < 3; ++a) {
+    w.ext[a] = (float)grid[a] * 256.0f;
+    w.ext_in[a] = w.ext[a] - 1e-3f;
+  }
+  w.roots = roots;
+  w.nodes = reinterpret_cast<const uint4*>(nodes);
+  w.bricks = reinterpret_cast<const uint32_t*>(bricks);
+  return w;
+}
 
 static DevParams make_params(const arc_trace_params& t) {
   DevParams p;
@@ -738,7 +817,7 @@ static DevParams make_params(const arc_trace_params& t) {
 
 template <uint32_t SEM, bool FULL>
 static const void* kernel_ptr() {
-  return reinterpret_cast<const void*>(&k_trace<SEM, FULL>);
+  return reinterpret_cast<const void*>(&k_trace<SEM, FULL, SEM == ARC_SEM_WORLD>);
 }
 
 hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_threads, uint32_t* blocks_per_cu) {
@@ -750,32 +829,24 @@ hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_thread
   return e;
 }
 
+template <uint32_t SEM, bool FULL, bool WATER>
+static void launch_one(const LaunchArgs& a, const DevWorld& w, const DevParams& p, hipStream_t stream) {
+  hipLaunchKernelGGL((k_trace<SEM, FULL, WATER>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p, a.rays,
+                     reinterpret_cast<typename OutPtr<FULL>::T*>(a.out), (uint32_t)a.n, a.queue, a.refill, a.batch);
+}
+
 hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
-  DevWorld w;
-  w.gx = a.grid[0];
-  w.gy = a.grid[1];
-  w.gz = a.grid[2];
-  w.pad = 0;
-  w.roots = a.roots;
-  w.nodes = reinterpret_cast<const uint4*>(a.nodes);
-  w.bricks = reinterpret_cast<const uint32_t*>(a.bricks);
+  const DevWorld w = make_world(a.grid, a.roots, a.nodes, a.bricks);
   const DevParams p = make_params(a.params);
-  const dim3 grid(a.blocks), block(a.block_threads);
-  const uint32_t n = (uint32_t)a.n;
   if (a.params.semantics == ARC_SEM_CHUNK) {
-    if (a.full)
-      hipLaunchKernelGGL((k_trace<ARC_SEM_CHUNK, true>), grid, block, 0, stream, w, p, a.rays,
-                         reinterpret_cast<arc_trace_result*>(a.out), n, a.queue, a.refill, a.batch);
-    else
-      hipLaunchKernelGGL((k_trace<ARC_SEM_CHUNK, false>), grid, block, 0, stream, w, p, a.rays,
-                         reinterpret_cast<arc_hit*>(a.out), n, a.queue, a.refill, a.batch);
+    if (a.full) launch_one<ARC_SEM_CHUNK, true, false>(a, w, p, stream);
+    else launch_one<ARC_SEM_CHUNK, false, false>(a, w, p, stream);
+  } else if (p.ignore_water) {
+    if (a.full) launch_one<ARC_SEM_WORLD, true, true>(a, w, p, stream);
+    else launch_one<ARC_SEM_WORLD, false, true>(a, w, p, stream);
   } else {
-    if (a.full)
-      hipLaunchKernelGGL((k_trace<ARC_SEM_WORLD, true>), grid, block, 0, stream, w, p, a.rays,
-                         reinterpret_cast<arc_trace_result*>(a.out), n, a.queue, a.refill, a.batch);
-    else
-      hipLaunchKernelGGL((k_trace<ARC_SEM_WORLD, false>), grid, block, 0, stream, w, p, a.rays,
-                         reinterpret_cast<arc_hit*>(a.out), n, a.queue, a.refill, a.batch);
+    if (a.full) launch_one<ARC_SEM_WORLD, true, false>(a, w, p, stream);
+    else launch_one<ARC_SEM_WORLD, false, false>(a, w, p, stream);
   }
   return hipGetLastError();
 }
@@ -789,14 +860,7 @@ hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {
 }
 
 hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream) {
-  DevWorld w;
-  w.gx = a.grid[0];
-  w.gy = a.grid[1];
-  w.gz = a.grid[2];
-  w.pad = 0;
-  w.roots = a.roots;
-  w.nodes = reinterpret_cast<const uint4*>(a.nodes);
-  w.bricks = reinterpret_cast<const uint32_t*>(a.bricks);
+  const DevWorld w = make_world(a.grid, a.roots, a.nodes, a.bricks);
   PathCtl pc;
   pc.views = a.views;
   pc.view_first = a.view_first;

@@ -8,6 +8,9 @@
 
 namespace arcdev {
 
+// Queue counters of one persistent launch: items claimed and finished workgroups.
+constexpr uint32_t kQueueWords = 2u;
+
 struct LaunchArgs {
   uint32_t grid[3];
   const uint32_t* roots;
@@ -18,7 +21,7 @@ struct LaunchArgs {
   void* out;
   uint64_t n;
   bool full;
-  uint32_t* queue;  // two u32 counters, zero between launches
+  uint32_t* queue;  // kQueueWords u32 counters, zero between launches
   uint32_t blocks;
   uint32_t block_threads;
   uint32_t refill;
@@ -46,7 +49,7 @@ struct PathLaunchArgs {
   float bounce_max_dist_m;
   uint64_t seed;
   arc_hit* out;      // (1 + n_bounces) * n_items records
-  uint32_t* queue;   // two u32 counters, zero between launches
+  uint32_t* queue;   // kQueueWords u32 counters, zero between launches
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream);
