@@ -10,11 +10,20 @@ from .native import (  # noqa: F401
     ARC_GEN_SPARSE,
     ARC_SEM_CHUNK,
     ARC_SEM_WORLD,
+    ARC_SHADOW_BLOCKED,
+    ARC_SHADOW_BUDGET_DARK,
+    ARC_SHADOW_BUDGET_LIT,
+    ARC_SHADOW_CLEAR,
+    ARC_SHADOW_DI,
+    ARC_SHADOW_EMITTER,
+    ARC_SHADOW_MISS_WORLD,
+    ARC_SHADOW_SUN,
     BRICK_DTYPE,
     HIT_DTYPE,
     NODE_DTYPE,
     RAY_DTYPE,
     RESULT_DTYPE,
+    SHADOW_REC_DTYPE,
     SHARC_PROBE_DTYPE,
     SHARC_SAMPLE_DTYPE,
     ArcError,
@@ -26,6 +35,7 @@ from .native import (  # noqa: F401
     NrcParams,
     PATH_SAMPLE_DTYPE,
     ShadeParams,
+    ShadowParams,
     Sharc,
     TraceParams,
     bounce_rays,

@@ -31,7 +31,7 @@ struct arc_ctx {
   uint64_t n_nodes = 0, node_cap = 0, n_bricks = 0, brick_cap = 0;
   bool has_world = false;
   // persistent-queue counters: words [0, kQueueWords) for the traversal kernel, [16, 16 +
-  // kQueueWords) for the path kernel
+  // kQueueWords) for the path kernel, [32, 32 + kQueueWords) for the shadow walkers
   uint32_t* d_queue = nullptr;
   // scratch for the host-pointer entry point
   arc_ray* d_rays = nullptr;
@@ -71,7 +71,7 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail((ctx), ARC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-static_assert(16u + arcdev::kQueueWords <= 64u, "queue words");
+static_assert(32u + arcdev::kQueueWords <= 64u, "queue words");
 
 // Traversal kernel, measured on MI355X for primary and diffuse rays: 6 workgroups of 256 per CU
 // (6 waves per SIMD; the occupancy limit of 8 is slower), refill at 16 idle lanes.
@@ -422,6 +422,43 @@ static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_r
   a.batch = cfg.batch_rays;
   hipError_t e = arcdev::launch_trace(a, (hipStream_t)stream);
   if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "trace launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
+int arc_trace_shadow_device(arc_ctx* c, const arc_shadow_params* sp, const arc_ray* d_rays, uint64_t n, float* d_vis,
+                            arc_shadow_rec* d_rec, void* stream) {
+  if (!c || !sp) return ARC_ERR_INVALID;
+  if (sp->mode != ARC_SHADOW_SUN && sp->mode != ARC_SHADOW_DI) return fail(c, ARC_ERR_INVALID, "unknown shadow mode");
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "trace before arc_world_upload");
+  if (n == 0) return ARC_OK;
+  if (!d_rays || !d_vis) return fail(c, ARC_ERR_INVALID, "null ray or visibility pointer");
+  if (n > 0x7FFFFFFFull) return fail(c, ARC_ERR_INVALID, "at most 2^31-1 rays per launch");
+  arcdev::ShadowLaunchArgs a{};
+  std::memcpy(a.grid, c->grid, sizeof a.grid);
+  a.roots = c->d_roots;
+  a.nodes = c->d_nodes;
+  a.bricks = c->d_bricks;
+  a.mode = sp->mode;
+  a.max_steps = sp->max_steps;
+  a.rays = d_rays;
+  a.vis = d_vis;
+  a.rec = d_rec;
+  a.n = n;
+  a.queue = c->d_queue + 32;  // the walkers' own counters
+  const arc_launch_config& cfg = c->cfg[0];
+  uint32_t bpc = cfg.blocks_per_cu;
+  if (bpc == 0) {
+    hipError_t oe = arcdev::shadow_occupancy(cfg.block_threads, &bpc);
+    if (oe != hipSuccess) return fail(c, ARC_ERR_HIP, "occupancy query: %s", hipGetErrorString(oe));
+  }
+  const uint64_t want_blocks = (uint64_t)bpc * c->n_cu;
+  const uint64_t max_useful = (n + cfg.block_threads - 1) / cfg.block_threads;
+  a.blocks = (uint32_t)std::max<uint64_t>(1u, std::min(want_blocks, max_useful));
+  a.block_threads = cfg.block_threads;
+  a.refill = cfg.refill_lanes;
+  a.batch = cfg.batch_rays;
+  hipError_t e = arcdev::launch_shadow(a, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "shadow launch: %s", hipGetErrorString(e));
   return ARC_OK;
 }
 

@@ -31,131 +31,10 @@
 #include "raymath.h"
 #include "trace_device.h"
 #include "trace_launch.h"
+#include "trace_walk.h"
 
 namespace arcdev {
 
-
-// packed per-lane flags
-enum : uint32_t {
-  F_CSHIFT = 0,  // 3 bits: cached_shift of the reference (level the last descent ended on)
-  F_SHIFT = 3,   // 3 bits: level of the hit cell (valid when done with a hit)
-  F_LIVE = 6,    // 1 bit: ray in flight
-  F_HIT = 7,     // 1 bit: finished with a hit
-  F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
-  F_DEPTH = 9,   // 3 bits: path kernel: 0 = primary ray, d = d-th bounce
-  F_DESC = 12,   // 1 bit: descent parked at the pass cap, continues at the F_CSHIFT level
-};
-
-__device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
-__device__ __forceinline__ uint32_t fput(uint32_t st, uint32_t off, uint32_t w, uint32_t v) {
-  const uint32_t m = ((1u << w) - 1u) << off;
-  return (st & ~m) | ((v << off) & m);
-}
-
-struct Lane {
-  float ox, oy, oz;  // origin, voxel units
-  float dx, dy, dz;  // direction used for inside points (WORLD: dir, CHUNK: safe_dir)
-  float ix, iy, iz;  // 1 / safe_dir
-  float nx, ny, nz;  // -origin * inv
-  float t, tl;       // t, t_limit
-  uint32_t vx, vy, vz;
-  uint32_t diff;     // prev_vi ^ vi of the coming iteration
-  uint32_t step;     // outer iterations begun
-  uint32_t st;       // packed flags
-  uint32_t mat;      // material of the hit
-  uint4 node;        // node of the level the last descent ended on
-  uint32_t idx;
-};
-
-// Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
-//   slot 0: chunk root node (after a hit: the node the hit resolved in),
-//   slot 1: cached level-1 ancestor, slot 2: cached level-2 ancestor (slot of shift s:
-//   (6 - s) / 2; the level-3 ancestor is the lane's `node`).
-constexpr uint32_t kMaxBlock = 256u;
-constexpr uint32_t kColdSlots = 3u;
-
-__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// float -> u32 for a value already clamped to x >= 0 (never NaN), whose result is then clamped
-// into a box below 2^32: min(x, 4294967040) (the largest float below 2^32) keeps the conversion
-// defined and gives the same clamped result as the saturating WGSL u32() / Rust `as`. Plain
-// arithmetic (v_min_f32 + v_cvt_u32_f32), so the compiler may speculate it into selects.
-__device__ __forceinline__ uint32_t cvt_u32(float x) { return (uint32_t)fminf(x, 4294967040.0f); }
-
-// t at which the ray crosses coordinate `c` on one axis: f32(c) * inv + (-o * inv).
-__device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
-  const float q = (float)c * inv;
-  return q + noi;
-}
-
-// LOD min_shift (traversal64.wgsl:355-370); `sidx` is the iteration index.
-template <uint32_t SEM>
-__device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, uint32_t sidx) {
-  if (SEM == ARC_SEM_CHUNK) return 0u;
-  uint32_t m = 0u;
-  if (p.lod_bias == 0u) {
-    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
-  } else if (p.lod_bias == 1u) {
-    if (t > p.lod_t_far || sidx > p.lod_step) m = 4u;
-    else if (t > p.lod_t) m = 2u;
-  } else {
-    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
-  }
-  return m;
-}
-
-// Guards at the top of an outer iteration (traversal64.wgsl:302-310 / raycast_cpu.rs:78-84).
-template <uint32_t SEM>
-__device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const DevParams& p, const Lane& L) {
-  const uint32_t max_steps = SEM == ARC_SEM_CHUNK ? 512u : p.max_steps;
-  const uint32_t wx = SEM == ARC_SEM_CHUNK ? 256u : (w.gx << 8);
-  const uint32_t wy = SEM == ARC_SEM_CHUNK ? 256u : (w.gy << 8);
-  const uint32_t wz = SEM == ARC_SEM_CHUNK ? 256u : (w.gz << 8);
-  return L.step >= max_steps || L.t >= L.tl || L.vx >= wx || L.vy >= wy || L.vz >= wz;
-}
-
-// Restart level of an iteration without a chunk change (traversal64.wgsl:374-386). The
-// reference tests c3 (diff < 4, min_shift == 0, cached_shift == 0), then c2 (diff < 16, both
-// <= 2), then c1 (diff < 64, both <= 4), else restarts at the chunk root. With m =
-// max(min_shift, cached_shift), both even, and dl the lowest level whose block holds both the old
-// and the new voxel (0 for diff < 4, 2 below 16, 4 below 64, else 6), the first condition that
-// holds is the one of level max(dl, m): that is the restart level. dl = bit length of diff,
-// minus one, rounded down to even, capped at 6.
-__device__ __forceinline__ uint32_t restart_shift(uint32_t diff, uint32_t minsh, uint32_t cs) {
-  const uint32_t dl = umin_(6u, (31u - (uint32_t)__builtin_clz(diff | 1u)) & ~1u);
-  return umax_(dl, umax_(minsh, cs));
-}
-
-// Address of the LDS slot of the cached node at shift `sh` (slot (6 - sh) / 2).
-__device__ __forceinline__ uint4* cold_slot(uint4* C, uint32_t sh) {
-  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (kMaxBlock * 8u));
-}
-
-// Chunk roots staged in LDS for worlds of at most kLdsChunks chunks: the root index and the root
-// node of every chunk, read at each chunk change in place of chunk_roots[key] followed by
-// nodes[root] (two dependent global loads on the longest dependency chain of a wave trip).
-constexpr uint32_t kLdsChunks = 256u;
-struct RootLds {
-  uint4 node[kLdsChunks];
-  uint32_t idx[kLdsChunks];
-};
-
-// Fills `R` from the pools; returns false (block-uniform) when the world has more chunks.
-__device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLds& R) {
-  const uint32_t nc = w.gx * w.gy * w.gz;
-  if (nc > kLdsChunks) return false;
-  for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
-    const uint32_t r = w.roots[c];
-    R.idx[c] = r;
-    R.node[c] = r == ARC_EMPTY_CHUNK_SENTINEL ? make_uint4(0u, 0u, 0u, 0u) : w.nodes[r];
-  }
-  __syncthreads();
-  return true;
-}
-
-// Descent passes (child-bit tests) one wave trip may run for a lane before it parks the descent
-// and continues it on the next trip; 4 = never park (a descent has at most four levels).
-constexpr uint32_t kMaxPasses = 4u;
 
 // One outer iteration of a live lane whose guards passed (traversal64.wgsl:311-552).
 // Instruction count is what bounds the kernel, so the sequence is written for few VALU ops:
@@ -165,10 +44,8 @@ constexpr uint32_t kMaxPasses = 4u;
 //   * the descent loop only tests and loads; the material of a hit candidate (a set bit in a
 //     leaf or at the LOD floor) is resolved after the loop;
 //   * cached_shift always equals the level the iteration ended on, so one field carries both;
-//   * the chunk root is read from the chunk_roots array on every chunk change (the array is a
-//     few KB and stays in cache), which gives the same root as the reference's 2-slot cache;
 //   * the box step computes the clamped inside coordinate of all three axes and selects, with no
-//     branches;
+//     branches (box_step);
 //   * WATER = false compiles the ignore_water tests out (ray classes that never set it).
 template <uint32_t SEM, bool WATER>
 __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C, const RootLds& R,
@@ -176,45 +53,15 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   uint32_t st = L.st;
   uint4 node = L.node;
   uint32_t sh = fget(st, F_CSHIFT, 3);
-  uint32_t span_mul = 1u;  // the box stepped over is span_mul << sh
-  bool descend = true;
-  uint32_t minsh;
-  if (kMaxPasses < 4u && fget(st, F_DESC, 1)) {  // continue a parked descent
-    minsh = lod_min_shift<SEM>(p, L.t, L.step - 1u);
-  } else {
-    const uint32_t sidx = L.step;
-    L.step = sidx + 1u;
-    minsh = lod_min_shift<SEM>(p, L.t, sidx);
-    const uint32_t diff = L.diff;
-    if (SEM == ARC_SEM_WORLD && diff >= 256u) {  // chunk change (:316-335)
-      const uint32_t key = (L.vx >> 8) + __umul24(L.vy >> 8, w.gx) + __umul24(L.vz >> 8, w.gx * w.gy);
-      uint32_t root;
-      if (lds_roots) {
-        root = R.idx[key];
-        node = R.node[key];
-      } else {
-        root = w.roots[key];
-        if (root != ARC_EMPTY_CHUNK_SENTINEL) node = w.nodes[root];
-      }
-      sh = 6u;
-      if (root == ARC_EMPTY_CHUNK_SENTINEL) {
-        descend = false;  // step over the empty chunk (:337-352)
-        span_mul = 4u;
-      } else {
-        C[0] = node;
-      }
-    } else {  // ancestor restart (:374-386)
-      const uint32_t rs = restart_shift(diff, minsh, sh);
-      if (rs != sh) node = *cold_slot(C, rs);
-      sh = rs;
-    }
-  }
+  const uint32_t sidx = L.step;
+  L.step = sidx + 1u;
+  const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
+  const bool descend = restart<SEM>(w, L, C, R, lds_roots, minsh, sh, node);
+  uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh (empty chunk: 256)
   bool cand = false;  // set bit at a leaf or at the LOD floor: hit unless the voxel is air/water
   if (descend) {
-    uint32_t loads = 0u;
     for (;;) {  // descent (:390-552), at most four levels
-      const uint32_t b = (__builtin_amdgcn_ubfe(L.vz, sh, 2) << 4) | (__builtin_amdgcn_ubfe(L.vy, sh, 2) << 2) |
-                         __builtin_amdgcn_ubfe(L.vx, sh, 2);
+      const uint32_t b = child_bit(L, sh);
       if (SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
           (node.w >> 24) == ARC_MAT_WATER) {
         span_mul = 4u;
@@ -231,16 +78,10 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
       sh -= 2u;
       if (sh != 0u) *cold_slot(C, sh) = node;
-      if (kMaxPasses < 4u && ++loads == kMaxPasses) {  // park: the next trip tests `node`
-        L.node = node;
-        L.st = fput(fput(st, F_CSHIFT, 3, sh), F_DESC, 1, 1u);
-        return;
-      }
     }
   }
   if (cand) {  // :441-538
-    const uint32_t b = (__builtin_amdgcn_ubfe(L.vz, sh, 2) << 4) | (__builtin_amdgcn_ubfe(L.vy, sh, 2) << 2) |
-                       __builtin_amdgcn_ubfe(L.vx, sh, 2);
+    const uint32_t b = child_bit(L, sh);
     uint32_t mat = node.w >> 24;
     if ((node.z & (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG)) == ARC_IS_LEAF_FLAG) {
       const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
@@ -250,123 +91,15 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
     if (!skip_water && mat != ARC_MAT_AIR) {  // hit: resolved at finalize (:467-523)
       L.mat = mat;
       L.node = node;
-      L.st = fput(fput(fput(fput(fput(st, F_CSHIFT, 3, sh), F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u),
-                  F_DESC, 1, 0u);
+      L.st = fput(fput(fput(fput(st, F_CSHIFT, 3, sh), F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
       return;
     }
   }
   L.node = node;
-  // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292)
-  const uint32_t span = span_mul << sh;
-  const uint32_t mk = 0u - span;
-  const uint32_t bx = L.vx & mk, by = L.vy & mk, bz = L.vz & mk;
-  const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
-  const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
-  const float tx = plane_t(ex, L.ix, L.nx), ty = plane_t(ey, L.iy, L.ny), tz = plane_t(ez, L.iz, L.nz);
-  const float t_exit = fminf(fminf(tx, ty), tz);
-  const float next_t = fmaxf(L.t, t_exit);
-  const bool is_x = tx <= ty && tx <= tz;
-  const bool is_y = !is_x && (ty <= tz);
-  const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
-  const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
-  const uint32_t last = span - 1u;
-  const uint32_t cx = umin_(umax_(cvt_u32(ipx), bx), bx + last);
-  const uint32_t cy = umin_(umax_(cvt_u32(ipy), by), by + last);
-  const uint32_t cz = umin_(umax_(cvt_u32(ipz), bz), bz + last);
-  const uint32_t fx = px ? ex : bx - 1u, fy = py ? ey : by - 1u, fz = pz ? ez : bz - 1u;
-  const uint32_t nvx = is_x ? fx : cx;
-  const uint32_t nvy = is_y ? fy : cy;
-  const uint32_t nvz = (is_x || is_y) ? cz : fz;
-  L.diff = (nvx ^ L.vx) | (nvy ^ L.vy) | (nvz ^ L.vz);
-  L.vx = nvx;
-  L.vy = nvy;
-  L.vz = nvz;
-  L.t = next_t;
-  st = fput(fput(st, F_CSHIFT, 3, sh), F_DESC, 1, 0u);
+  box_step(L, span_mul << sh);
+  st = fput(st, F_CSHIFT, 3, sh);
   if (iteration_guards_fail<SEM>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
   L.st = st;
-}
-
-// Ray setup: traversal64.wgsl:234-300 (WORLD) / raycast_cpu.rs:34-76 (CHUNK), then the guards
-// of iteration 0.
-template <uint32_t SEM>
-__device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, const arc_ray& r, Lane& L,
-                                         uint4* C) {
-  L.step = 0u;
-  L.st = fput(0u, F_HASRAY, 1, 1u);  // not live until the setup succeeds
-  if (SEM == ARC_SEM_WORLD) {
-    const float max_t_v = r.t_max * 16.0f;
-    const float start_t_v = fmaxf(r.t_start * 16.0f, 0.0f);
-    L.ox = r.ox * 16.0f;
-    L.oy = r.oy * 16.0f;
-    L.oz = r.oz * 16.0f;
-    L.dx = r.dx;
-    L.dy = r.dy;
-    L.dz = r.dz;
-    const float sx = fabsf(r.dx) < 1e-7f ? 1e-7f : r.dx;
-    const float sy = fabsf(r.dy) < 1e-7f ? 1e-7f : r.dy;
-    const float sz = fabsf(r.dz) < 1e-7f ? 1e-7f : r.dz;
-    L.ix = 1.0f / sx;
-    L.iy = 1.0f / sy;
-    L.iz = 1.0f / sz;
-    const float wx = w.ext[0], wy = w.ext[1], wz = w.ext[2];
-    // intersect_aabb (traversal64.wgsl:134-142)
-    const float tnx = ((L.ix > 0.0f ? 0.0f : wx) - L.ox) * L.ix;
-    const float tny = ((L.iy > 0.0f ? 0.0f : wy) - L.oy) * L.iy;
-    const float tnz = ((L.iz > 0.0f ? 0.0f : wz) - L.oz) * L.iz;
-    const float txx = ((L.ix > 0.0f ? wx : 0.0f) - L.ox) * L.ix;
-    const float txy = ((L.iy > 0.0f ? wy : 0.0f) - L.oy) * L.iy;
-    const float txz = ((L.iz > 0.0f ? wz : 0.0f) - L.oz) * L.iz;
-    const float wh_x = fmaxf(fmaxf(tnx, tny), tnz);
-    const float wh_y = fminf(fminf(txx, txy), txz);
-    const float t_enter = fmaxf(wh_x, start_t_v);
-    if (wh_y < t_enter || t_enter >= max_t_v) return;
-    L.t = t_enter;
-    L.tl = fminf(wh_y, max_t_v);
-    const float tt = L.t + 1e-4f;
-    const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
-    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, w.ext_in[0]));
-    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, w.ext_in[1]));
-    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, w.ext_in[2]));
-  } else {
-    const uint32_t root = w.roots[0];
-    if (root == ARC_EMPTY_CHUNK_SENTINEL) return;
-    const uint4 rn = w.nodes[root];
-    if (rn.x == 0u && rn.y == 0u) return;  // tree.is_empty()
-    L.ox = r.ox;
-    L.oy = r.oy;
-    L.oz = r.oz;
-    L.dx = fabsf(r.dx) > 1e-7f ? r.dx : copysignf(1e-7f, r.dx);
-    L.dy = fabsf(r.dy) > 1e-7f ? r.dy : copysignf(1e-7f, r.dy);
-    L.dz = fabsf(r.dz) > 1e-7f ? r.dz : copysignf(1e-7f, r.dz);
-    L.ix = 1.0f / L.dx;
-    L.iy = 1.0f / L.dy;
-    L.iz = 1.0f / L.dz;
-    // intersect_aabb (raycast_cpu.rs:222-234)
-    const float ax0 = (0.0f - L.ox) * L.ix, ax1 = (256.0f - L.ox) * L.ix;
-    const float ay0 = (0.0f - L.oy) * L.iy, ay1 = (256.0f - L.oy) * L.iy;
-    const float az0 = (0.0f - L.oz) * L.iz, az1 = (256.0f - L.oz) * L.iz;
-    const float tmin = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fminf(az0, az1));
-    const float tmax = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1));
-    if (!(tmax >= fmaxf(tmin, 0.0f))) return;
-    const float t_enter = fmaxf(tmin, 0.0f);
-    if (tmax < t_enter || t_enter >= r.t_max) return;
-    L.t = t_enter;
-    L.tl = fminf(tmax, r.t_max);
-    const float tt = L.t + 1e-4f;
-    const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
-    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, 256.0f - 1e-3f));
-    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, 256.0f - 1e-3f));
-    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, 256.0f - 1e-3f));
-    C[0] = rn;    // chunk root for every root restart
-    L.node = rn;  // `node` == cached ancestor at cached_shift (6): iteration 0 restarts here
-  }
-  L.nx = (-L.ox) * L.ix;
-  L.ny = (-L.oy) * L.iy;
-  L.nz = (-L.oz) * L.iz;
-  L.diff = 0xFFFFFFFFu;  // prev_vi starts at u32::MAX
-  if (iteration_guards_fail<SEM>(w, p, L)) return;
-  L.st = fput(fput(L.st, F_CSHIFT, 3, 6u), F_LIVE, 1, 1u);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -783,23 +516,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // Launchers
 // ---------------------------------------------------------------------------------------
 namespace arcdev {
-
-static DevWorld make_world(const uint32_t grid[3], const uint32_t* roots, const arc_node64* nodes,
-                           const arc_brick64* bricks) {
-  DevWorld w;
-  w.gx = grid[0];
-  w.gy = grid[1];
-  w.gz = grid[2];
-  w.pad = 0;
-  for (int a = 0; a < 3; ++a) {
-    w.ext[a] = (float)grid[a] * 256.0f;
-    w.ext_in[a] = w.ext[a] - 1e-3f;
-  }
-  w.roots = roots;
-  w.nodes = reinterpret_cast<const uint4*>(nodes);
-  w.bricks = reinterpret_cast<const uint32_t*>(bricks);
-  return w;
-}
 
 static DevParams make_params(const arc_trace_params& t) {
   DevParams p;

@@ -53,6 +53,23 @@ struct PathLaunchArgs {
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream);
+
+// One launch of a shadow any-hit walker over a ray set.
+struct ShadowLaunchArgs {
+  uint32_t grid[3];
+  const uint32_t* roots;
+  const arc_node64* nodes;
+  const arc_brick64* bricks;
+  uint32_t mode, max_steps;  // arc_shadow_params
+  const arc_ray* rays;
+  float* vis;
+  arc_shadow_rec* rec;  // may be null
+  uint64_t n;
+  uint32_t* queue;  // kQueueWords u32 counters, zero between launches
+  uint32_t blocks, block_threads, refill, batch;
+};
+hipError_t launch_shadow(const ShadowLaunchArgs& a, hipStream_t stream);
+hipError_t shadow_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu);
 hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu);
 // Resident workgroups per CU of the traversal kernel at `block_threads`.
 hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_threads, uint32_t* blocks_per_cu);

@@ -1,0 +1,305 @@
+// Per-lane state and building blocks shared by the traversal walkers (k_trace, k_path,
+// k_shadow): packed lane flags, the LDS cold slots of cached ancestors, chunk roots staged in LDS,
+// the closed-form restart level and the exact box step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "trace_device.h"
+
+namespace arcdev {
+
+// packed per-lane flags
+enum : uint32_t {
+  F_CSHIFT = 0,  // 3 bits: cached_shift of the reference (level the last descent ended on)
+  F_SHIFT = 3,   // 3 bits: level of the hit cell (valid when done with a hit)
+  F_LIVE = 6,    // 1 bit: ray in flight
+  F_HIT = 7,     // 1 bit: finished with a hit
+  F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
+  F_DEPTH = 9,   // 3 bits: path kernel: 0 = primary ray, d = d-th bounce
+};
+
+__device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
+__device__ __forceinline__ uint32_t fput(uint32_t st, uint32_t off, uint32_t w, uint32_t v) {
+  const uint32_t m = ((1u << w) - 1u) << off;
+  return (st & ~m) | ((v << off) & m);
+}
+
+struct Lane {
+  float ox, oy, oz;  // origin, voxel units
+  float dx, dy, dz;  // direction used for inside points (WORLD: dir, CHUNK: safe_dir)
+  float ix, iy, iz;  // 1 / safe_dir
+  float nx, ny, nz;  // -origin * inv
+  float t, tl;       // t, t_limit
+  uint32_t vx, vy, vz;
+  uint32_t diff;     // prev_vi ^ vi of the coming iteration
+  uint32_t step;     // outer iterations begun
+  uint32_t st;       // packed flags
+  uint32_t mat;      // material of the hit
+  uint4 node;        // node of the level the last descent ended on
+  uint32_t idx;
+};
+
+// Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
+//   slot 0: chunk root node (after a hit: the node the hit resolved in),
+//   slot 1: cached level-1 ancestor, slot 2: cached level-2 ancestor (slot of shift s:
+//   (6 - s) / 2; the level-3 ancestor is the lane's `node`).
+constexpr uint32_t kMaxBlock = 256u;
+constexpr uint32_t kColdSlots = 3u;
+
+__device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// float -> u32 for a value already clamped to x >= 0 (never NaN), whose result is then clamped
+// into a box below 2^32: min(x, 4294967040) (the largest float below 2^32) keeps the conversion
+// defined and gives the same clamped result as the saturating WGSL u32() / Rust `as`. Plain
+// arithmetic (v_min_f32 + v_cvt_u32_f32), so the compiler may speculate it into selects.
+__device__ __forceinline__ uint32_t cvt_u32(float x) { return (uint32_t)fminf(x, 4294967040.0f); }
+
+// t at which the ray crosses coordinate `c` on one axis: f32(c) * inv + (-o * inv).
+__device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
+  const float q = (float)c * inv;
+  return q + noi;
+}
+
+// LOD min_shift (traversal64.wgsl:355-370); `sidx` is the iteration index.
+template <uint32_t SEM>
+__device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, uint32_t sidx) {
+  if (SEM == ARC_SEM_CHUNK) return 0u;
+  uint32_t m = 0u;
+  if (p.lod_bias == 0u) {
+    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
+  } else if (p.lod_bias == 1u) {
+    if (t > p.lod_t_far || sidx > p.lod_step) m = 4u;
+    else if (t > p.lod_t) m = 2u;
+  } else {
+    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
+  }
+  return m;
+}
+
+// Guards at the top of an outer iteration (traversal64.wgsl:302-310 / raycast_cpu.rs:78-84).
+template <uint32_t SEM>
+__device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const DevParams& p, const Lane& L) {
+  const uint32_t max_steps = SEM == ARC_SEM_CHUNK ? 512u : p.max_steps;
+  const uint32_t wx = SEM == ARC_SEM_CHUNK ? 256u : (w.gx << 8);
+  const uint32_t wy = SEM == ARC_SEM_CHUNK ? 256u : (w.gy << 8);
+  const uint32_t wz = SEM == ARC_SEM_CHUNK ? 256u : (w.gz << 8);
+  return L.step >= max_steps || L.t >= L.tl || L.vx >= wx || L.vy >= wy || L.vz >= wz;
+}
+
+// Restart level of an iteration without a chunk change (traversal64.wgsl:374-386). The
+// reference tests c3 (diff < 4, min_shift == 0, cached_shift == 0), then c2 (diff < 16, both
+// <= 2), then c1 (diff < 64, both <= 4), else restarts at the chunk root. With m =
+// max(min_shift, cached_shift), both even, and dl the lowest level whose block holds both the old
+// and the new voxel (0 for diff < 4, 2 below 16, 4 below 64, else 6), the first condition that
+// holds is the one of level max(dl, m): that is the restart level. dl = bit length of diff,
+// minus one, rounded down to even, capped at 6.
+__device__ __forceinline__ uint32_t restart_shift(uint32_t diff, uint32_t minsh, uint32_t cs) {
+  const uint32_t dl = umin_(6u, (31u - (uint32_t)__builtin_clz(diff | 1u)) & ~1u);
+  return umax_(dl, umax_(minsh, cs));
+}
+
+// Address of the LDS slot of the cached node at shift `sh` (slot (6 - sh) / 2).
+__device__ __forceinline__ uint4* cold_slot(uint4* C, uint32_t sh) {
+  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (kMaxBlock * 8u));
+}
+
+// Chunk roots staged in LDS for worlds of at most kLdsChunks chunks: the root index and the root
+// node of every chunk, read at each chunk change in place of chunk_roots[key] followed by
+// nodes[root] (two dependent global loads on the longest dependency chain of a wave trip).
+constexpr uint32_t kLdsChunks = 256u;
+struct RootLds {
+  uint4 node[kLdsChunks];
+  uint32_t idx[kLdsChunks];
+};
+
+// Fills `R` from the pools; returns false (block-uniform) when the world has more chunks.
+__device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLds& R) {
+  const uint32_t nc = w.gx * w.gy * w.gz;
+  if (nc > kLdsChunks) return false;
+  for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+    const uint32_t r = w.roots[c];
+    R.idx[c] = r;
+    R.node[c] = r == ARC_EMPTY_CHUNK_SENTINEL ? make_uint4(0u, 0u, 0u, 0u) : w.nodes[r];
+  }
+  __syncthreads();
+  return true;
+}
+
+// Restart of an outer iteration (traversal64.wgsl:312-386; the shadow walkers repeat it at
+// :620-669 / :830-879). On a chunk change (diff >= 256) the node is the chunk's root, read from
+// chunk_roots on every change (the reference's 2-slot cache only saves those reads, the root is
+// the same) or from the LDS copy; an empty chunk returns false (the caller steps over 256
+// voxels). Otherwise the restart level is restart_shift() and the node the cached ancestor
+// there. On return `sh` is the level the descent starts on and `node` the node at that level.
+template <uint32_t SEM>
+__device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4* C, const RootLds& R, bool lds_roots,
+                                        uint32_t minsh, uint32_t& sh, uint4& node) {
+  if (SEM == ARC_SEM_WORLD && L.diff >= 256u) {
+    const uint32_t key = (L.vx >> 8) + __umul24(L.vy >> 8, w.gx) + __umul24(L.vz >> 8, w.gx * w.gy);
+    uint32_t root;
+    if (lds_roots) {
+      root = R.idx[key];
+      node = R.node[key];
+    } else {
+      root = w.roots[key];
+      if (root != ARC_EMPTY_CHUNK_SENTINEL) node = w.nodes[root];
+    }
+    sh = 6u;
+    if (root == ARC_EMPTY_CHUNK_SENTINEL) return false;
+    C[0] = node;
+    return true;
+  }
+  const uint32_t rs = restart_shift(L.diff, minsh, sh);
+  if (rs != sh) node = *cold_slot(C, rs);
+  sh = rs;
+  return true;
+}
+
+// step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292) over the box of `span`
+// voxels that holds the lane's voxel: t moves to the box exit, the voxel to the first one beyond
+// it (exit axis) or to the clamped inside point (other axes), and `diff` records what changed.
+// All three axes are computed and selected, with no branches.
+__device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
+  const uint32_t mk = 0u - span;
+  const uint32_t bx = L.vx & mk, by = L.vy & mk, bz = L.vz & mk;
+  const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
+  const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
+  const float tx = plane_t(ex, L.ix, L.nx), ty = plane_t(ey, L.iy, L.ny), tz = plane_t(ez, L.iz, L.nz);
+  const float t_exit = fminf(fminf(tx, ty), tz);
+  const float next_t = fmaxf(L.t, t_exit);
+  const bool is_x = tx <= ty && tx <= tz;
+  const bool is_y = !is_x && (ty <= tz);
+  const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
+  const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
+  const uint32_t last = span - 1u;
+  const uint32_t cx = umin_(umax_(cvt_u32(ipx), bx), bx + last);
+  const uint32_t cy = umin_(umax_(cvt_u32(ipy), by), by + last);
+  const uint32_t cz = umin_(umax_(cvt_u32(ipz), bz), bz + last);
+  const uint32_t fx = px ? ex : bx - 1u, fy = py ? ey : by - 1u, fz = pz ? ez : bz - 1u;
+  const uint32_t nvx = is_x ? fx : cx;
+  const uint32_t nvy = is_y ? fy : cy;
+  const uint32_t nvz = (is_x || is_y) ? cz : fz;
+  L.diff = (nvx ^ L.vx) | (nvy ^ L.vy) | (nvz ^ L.vz);
+  L.vx = nvx;
+  L.vy = nvy;
+  L.vz = nvz;
+  L.t = next_t;
+}
+
+// Child slot b of the voxel at shift `sh` (traversal64.wgsl:391-394): (z << 4) | (y << 2) | x of
+// the voxel's 2-bit digits at that level.
+__device__ __forceinline__ uint32_t child_bit(const Lane& L, uint32_t sh) {
+  return (__builtin_amdgcn_ubfe(L.vz, sh, 2) << 4) | (__builtin_amdgcn_ubfe(L.vy, sh, 2) << 2) |
+         __builtin_amdgcn_ubfe(L.vx, sh, 2);
+}
+
+// Marker a shadow walker's `mat` holds once its setup found the world entry (before that it
+// holds ARC_SHADOW_MISS_WORLD; after an in-loop return, the outcome).
+constexpr uint32_t kEntered = 0xFFu;
+
+// Ray setup: traversal64.wgsl:234-300 (WORLD) / raycast_cpu.rs:34-76 (CHUNK), then the guards
+// of iteration 0. SHADOW: the setup of trace_shadow_vis / _di (:576-605 / :786-815), which
+// starts at the world entry (t_start unused) and misses when the entry lies at or beyond
+// max_dist (world_hit.x >= max_t_v) rather than when max(entry, 0) does.
+template <uint32_t SEM, bool SHADOW = false>
+__device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, const arc_ray& r, Lane& L,
+                                         uint4* C) {
+  L.step = 0u;
+  L.st = fput(0u, F_HASRAY, 1, 1u);  // not live until the setup succeeds
+  if (SEM == ARC_SEM_WORLD) {
+    const float max_t_v = r.t_max * 16.0f;
+    const float start_t_v = SHADOW ? 0.0f : fmaxf(r.t_start * 16.0f, 0.0f);
+    L.ox = r.ox * 16.0f;
+    L.oy = r.oy * 16.0f;
+    L.oz = r.oz * 16.0f;
+    L.dx = r.dx;
+    L.dy = r.dy;
+    L.dz = r.dz;
+    const float sx = fabsf(r.dx) < 1e-7f ? 1e-7f : r.dx;
+    const float sy = fabsf(r.dy) < 1e-7f ? 1e-7f : r.dy;
+    const float sz = fabsf(r.dz) < 1e-7f ? 1e-7f : r.dz;
+    L.ix = 1.0f / sx;
+    L.iy = 1.0f / sy;
+    L.iz = 1.0f / sz;
+    const float wx = w.ext[0], wy = w.ext[1], wz = w.ext[2];
+    // intersect_aabb (traversal64.wgsl:134-142)
+    const float tnx = ((L.ix > 0.0f ? 0.0f : wx) - L.ox) * L.ix;
+    const float tny = ((L.iy > 0.0f ? 0.0f : wy) - L.oy) * L.iy;
+    const float tnz = ((L.iz > 0.0f ? 0.0f : wz) - L.oz) * L.iz;
+    const float txx = ((L.ix > 0.0f ? wx : 0.0f) - L.ox) * L.ix;
+    const float txy = ((L.iy > 0.0f ? wy : 0.0f) - L.oy) * L.iy;
+    const float txz = ((L.iz > 0.0f ? wz : 0.0f) - L.oz) * L.iz;
+    const float wh_x = fmaxf(fmaxf(tnx, tny), tnz);
+    const float wh_y = fminf(fminf(txx, txy), txz);
+    const float t_enter = fmaxf(wh_x, start_t_v);
+    if (wh_y < t_enter || (SHADOW ? wh_x : t_enter) >= max_t_v) return;
+    if (SHADOW) L.mat = kEntered;
+    L.t = t_enter;
+    L.tl = fminf(wh_y, max_t_v);
+    const float tt = L.t + 1e-4f;
+    const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
+    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, w.ext_in[0]));
+    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, w.ext_in[1]));
+    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, w.ext_in[2]));
+  } else {
+    const uint32_t root = w.roots[0];
+    if (root == ARC_EMPTY_CHUNK_SENTINEL) return;
+    const uint4 rn = w.nodes[root];
+    if (rn.x == 0u && rn.y == 0u) return;  // tree.is_empty()
+    L.ox = r.ox;
+    L.oy = r.oy;
+    L.oz = r.oz;
+    L.dx = fabsf(r.dx) > 1e-7f ? r.dx : copysignf(1e-7f, r.dx);
+    L.dy = fabsf(r.dy) > 1e-7f ? r.dy : copysignf(1e-7f, r.dy);
+    L.dz = fabsf(r.dz) > 1e-7f ? r.dz : copysignf(1e-7f, r.dz);
+    L.ix = 1.0f / L.dx;
+    L.iy = 1.0f / L.dy;
+    L.iz = 1.0f / L.dz;
+    // intersect_aabb (raycast_cpu.rs:222-234)
+    const float ax0 = (0.0f - L.ox) * L.ix, ax1 = (256.0f - L.ox) * L.ix;
+    const float ay0 = (0.0f - L.oy) * L.iy, ay1 = (256.0f - L.oy) * L.iy;
+    const float az0 = (0.0f - L.oz) * L.iz, az1 = (256.0f - L.oz) * L.iz;
+    const float tmin = fmaxf(fmaxf(fminf(ax0, ax1), fminf(ay0, ay1)), fminf(az0, az1));
+    const float tmax = fminf(fminf(fmaxf(ax0, ax1), fmaxf(ay0, ay1)), fmaxf(az0, az1));
+    if (!(tmax >= fmaxf(tmin, 0.0f))) return;
+    const float t_enter = fmaxf(tmin, 0.0f);
+    if (tmax < t_enter || t_enter >= r.t_max) return;
+    L.t = t_enter;
+    L.tl = fminf(tmax, r.t_max);
+    const float tt = L.t + 1e-4f;
+    const float qx = L.dx * tt, qy = L.dy * tt, qz = L.dz * tt;
+    L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, 256.0f - 1e-3f));
+    L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, 256.0f - 1e-3f));
+    L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, 256.0f - 1e-3f));
+    C[0] = rn;    // chunk root for every root restart
+    L.node = rn;  // `node` == cached ancestor at cached_shift (6): iteration 0 restarts here
+  }
+  L.nx = (-L.ox) * L.ix;
+  L.ny = (-L.oy) * L.iy;
+  L.nz = (-L.oz) * L.iz;
+  L.diff = 0xFFFFFFFFu;  // prev_vi starts at u32::MAX
+  if (iteration_guards_fail<SEM>(w, p, L)) return;
+  L.st = fput(fput(L.st, F_CSHIFT, 3, 6u), F_LIVE, 1, 1u);
+}
+
+// Device view of the pools (host side of a launch).
+inline DevWorld make_world(const uint32_t grid[3], const uint32_t* roots, const arc_node64* nodes,
+                           const arc_brick64* bricks) {
+  DevWorld w;
+  w.gx = grid[0];
+  w.gy = grid[1];
+  w.gz = grid[2];
+  w.pad = 0;
+  for (int a = 0; a < 3; ++a) {
+    w.ext[a] = (float)grid[a] * 256.0f;
+    w.ext_in[a] = w.ext[a] - 1e-3f;
+  }
+  w.roots = roots;
+  w.nodes = reinterpret_cast<const uint4*>(nodes);
+  w.bricks = reinterpret_cast<const uint32_t*>(bricks);
+  return w;
+}
+
+}  // namespace arcdev

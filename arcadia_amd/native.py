@@ -27,6 +27,14 @@ ARC_GEN_SPARSE = 2
 ARC_FACE_NONE = 255
 ARC_HIT_FLAG_HIT = 1
 ARC_HIT_FLAG_LOD = 2
+ARC_SHADOW_SUN = 0
+ARC_SHADOW_DI = 1
+ARC_SHADOW_MISS_WORLD = 0
+ARC_SHADOW_CLEAR = 1
+ARC_SHADOW_BLOCKED = 2
+ARC_SHADOW_EMITTER = 3
+ARC_SHADOW_BUDGET_DARK = 4
+ARC_SHADOW_BUDGET_LIT = 5
 ARC_IS_LEAF_FLAG = 0x80000000
 ARC_IS_UNIFORM_FLAG = 0x40000000
 ARC_PTR_INDEX_MASK = 0x3FFFFFFF
@@ -45,6 +53,16 @@ RESULT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("norm
                          ("reserved", "<u4", (2,))])
 assert NODE_DTYPE.itemsize == 16 and BRICK_DTYPE.itemsize == 64
 assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and RESULT_DTYPE.itemsize == 80
+
+
+SHADOW_REC_DTYPE = np.dtype([("t_voxels", "<f4"), ("steps", "<u4"), ("vx", "<u2"), ("vy", "<u2"), ("vz", "<u2"),
+                             ("outcome", "u1"), ("reserved", "u1")])
+
+
+class ShadowParams(C.Structure):
+    """arc_shadow_params: mode ARC_SHADOW_SUN (trace_shadow_vis) / ARC_SHADOW_DI (trace_shadow_vis_di),
+    max_steps = the quality's shadow_max_steps (72 at high quality)."""
+    _fields_ = [("mode", C.c_uint32), ("max_steps", C.c_uint32)]
 
 
 class TraceParams(C.Structure):
@@ -190,6 +208,8 @@ _SIGS = [
     ("arc_world_read_pools", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                        C.c_uint64, C.c_uint64]),
     ("arc_trace", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("arc_trace_shadow_device", C.c_int, [C.c_void_p, C.POINTER(ShadowParams), C.c_void_p, C.c_uint64, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
     ("arc_trace_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
                                    C.c_void_p]),
     ("arc_trace_full_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
@@ -701,6 +721,13 @@ class Context:
         """Device pointers (e.g. torch tensor data_ptr()), asynchronous on `stream`."""
         self._check(lib().arc_trace_device(self._h, C.byref(params), d_rays, n, d_hits, stream or None),
                     "arc_trace_device")
+
+    def trace_shadow_device(self, mode: int, max_steps: int, d_rays: int, n: int, d_vis: int, d_rec: int = 0,
+                            stream: int = 0) -> None:
+        """Shadow any-hit walker over device rays: d_vis f32 per ray, d_rec (optional) SHADOW_REC_DTYPE."""
+        sp = ShadowParams(mode, max_steps)
+        self._check(lib().arc_trace_shadow_device(self._h, C.byref(sp), d_rays, n, d_vis, d_rec or None,
+                                                  stream or None), "arc_trace_shadow_device")
 
     def trace_full_device(self, params: TraceParams, d_rays: int, n: int, d_out: int, stream: int = 0) -> None:
         self._check(lib().arc_trace_full_device(self._h, C.byref(params), d_rays, n, d_out, stream or None),

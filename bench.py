@@ -171,6 +171,60 @@ def batched_leg(args, threads, stream):
     return out
 
 
+def shadow_leg(args, ctx, pools, prays, phits, stream, threads):
+    """Shadow any-hit walkers on the full-frame world: one sun visibility ray per primary hit
+    (trace_shadow_vis, 72 steps, 88 m) and one visibility ray per hit towards another pixel's hit
+    point (trace_shadow_vis_di); device records checked against the oracle on a sample."""
+    import torch
+
+    import arcadia_amd as A
+    from arcadia_amd import shadow_rays as S
+
+    sun = (0.35, 0.8, 0.45)
+    out = {"metric": "Grays/s shadow any-hit walkers (full-frame world, rays from the 1080p primary hits)"}
+    sptr = stream.cuda_stream
+    for name, mode, rays in (("sun", A.ARC_SHADOW_SUN, S.sun_rays(prays, phits, sun, seed=args.seed)),
+                             ("di", A.ARC_SHADOW_DI, S.toward_rays(prays, phits, seed=args.seed))):
+        n = len(rays)
+        d_r = torch.from_numpy(rays.view(np.uint8)).cuda()
+        d_v = torch.empty(n, dtype=torch.float32, device="cuda")
+        d_rec = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+        for _ in range(2):
+            ctx.trace_shadow_device(mode, 72, d_r.data_ptr(), n, d_v.data_ptr(), d_rec.data_ptr(), sptr)
+        k = max(3, min(args.steps, 10))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(k):
+            ctx.trace_shadow_device(mode, 72, d_r.data_ptr(), n, d_v.data_ptr(), d_rec.data_ptr(), sptr)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / k
+        vis = d_v.cpu().numpy()
+        rec = d_rec.cpu().numpy().view(A.SHADOW_REC_DTYPE)
+        sample = np.arange(0, n, max(1, n // 8192))
+        exact = None
+        if not args.no_cpu_baseline:
+            from oracle import oracle as O
+
+            ovis, orec = O.trace_shadow(O.World(*pools), mode, 72, rays[sample], threads)
+            exact = bool(np.array_equal(ovis, vis[sample]) and
+                         np.array_equal(orec.view(np.uint32), rec[sample].view(np.uint32)))
+        out[name] = {
+            "rays": int(n),
+            "ms": round(ms, 4),
+            "grays": round(n / (ms * 1e-3) / 1e9, 4),
+            "visible_fraction": round(float(vis.mean()), 4),
+            "outcomes": np.bincount(rec["outcome"], minlength=6).tolist(),
+            "mean_steps": round(float(rec["steps"].mean()), 2),
+            "oracle_sample_exact": exact,
+            "sample_rays": int(len(sample)),
+        }
+        print(f"[bench] shadow {name}: {n} rays {ms:.3f} ms -> {out[name]['grays']} Grays/s, exact sample {exact}",
+              file=sys.stderr, flush=True)
+    return out
+
+
 def cache_leg(args, threads):
     """Learned radiance cache vs SHARC (2^22 entries) vs no cache, rel-MSE against the FP64
     1024-spp path-traced judge at 1920x1080 on the full-frame world with 8 emissive clusters."""
@@ -346,6 +400,7 @@ def main() -> int:
                          "autotuner still runs, so the timed launches use the same shapes as a full run")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shapes")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
+    ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-walker workload")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache workload")
     ap.add_argument("--cache-ref-spp", type=int, default=1024)
     ap.add_argument("--cache-frames", type=int, default=60)
@@ -361,6 +416,7 @@ def main() -> int:
     if args.profile:
         args.steps, args.warmup, args.no_cpu_baseline = min(args.steps, 5), 1, True
         args.no_batched = True
+        args.no_shadow = True
         args.no_cache = True
         args.no_destruction = True
 
@@ -489,6 +545,8 @@ def main() -> int:
                            steps_per_ray=round(bytes_per["steps_bounce"], 2)),
         }
         out["cpu_baseline"] = cpu
+    if not args.no_shadow:
+        out["shadow"] = shadow_leg(args, ctx, pools, prays, phits, stream, threads)
     ctx.close()
     del d_prays, d_phits, d_brays, d_bhits
     if not args.no_batched:

@@ -194,6 +194,42 @@ int arc_trace_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray
 /* Full TraceResult records (hit position, normal, lod rgb, voxel hash, contact AO). */
 int arc_trace_full_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
                           uint64_t n, arc_trace_result* d_out, void* stream);
+
+/* ---- shadow any-hit walkers ------------------------------------------------------------- */
+/* trace_shadow_vis (shaders/common/traversal64.wgsl:571-773): sun / point-light visibility,
+ * 1.0 unoccluded, 0.0 occluded. trace_shadow_vis_di (:781-989): visibility of a sampled
+ * emitter, emissive voxels within 16 voxels of the target distance count as unoccluded. Both
+ * read origin_m = (ox, oy, oz), dir = (dx, dy, dz), max_dist_m = t_max of each arc_ray; the
+ * reference walkers start at the world entry, so t_start is ignored. */
+#define ARC_SHADOW_SUN 0u /* trace_shadow_vis */
+#define ARC_SHADOW_DI 1u  /* trace_shadow_vis_di */
+typedef struct arc_shadow_params {
+  uint32_t mode;      /* ARC_SHADOW_SUN or ARC_SHADOW_DI */
+  uint32_t max_steps; /* u_cam.ray_steps.w = quality shadow_max_steps (72 at high quality);
+                         ARC_SHADOW_DI clamps it to 32..48 (:803) */
+} arc_shadow_params;
+/* How a walker ended (the reference's return paths). */
+#define ARC_SHADOW_MISS_WORLD 0u  /* ray misses the world box or starts past max_dist: 1.0 */
+#define ARC_SHADOW_CLEAR 1u       /* reached max_dist or left the world: 1.0 */
+#define ARC_SHADOW_BLOCKED 2u     /* solid cell at the walker's LOD level: 0.0 */
+#define ARC_SHADOW_EMITTER 3u     /* DI: emissive cell within 16 voxels of max_dist: 1.0 */
+#define ARC_SHADOW_BUDGET_DARK 4u /* step budget spent: SUN below 32 m height / DI short of the target: 0.0 */
+#define ARC_SHADOW_BUDGET_LIT 5u  /* step budget spent: SUN at or above 32 m / DI within 2 voxels: 1.0 */
+/* Per-ray record of a walker (no reference counterpart: the reference returns only the
+ * visibility). t_voxels = the walker's t when it returned (max_dist when it left the world),
+ * voxel = its current voxel then, steps = outer iterations begun. 16 B. */
+typedef struct arc_shadow_rec {
+  float t_voxels;
+  uint32_t steps;
+  uint16_t vx, vy, vz;
+  uint8_t outcome; /* ARC_SHADOW_* */
+  uint8_t reserved;
+} arc_shadow_rec;
+/* Device pointers, asynchronous on `stream`: d_vis[i] = visibility of ray i (0.0 / 1.0);
+ * d_rec may be NULL. */
+int arc_trace_shadow_device(arc_ctx* ctx, const arc_shadow_params* params, const arc_ray* d_rays, uint64_t n,
+                            float* d_vis, arc_shadow_rec* d_rec, void* stream);
+
 /* ---- primary G-buffer pass -------------------------------------------------------------- */
 /* Camera and frame inputs of primary_raymarch (the CameraUniforms fields it reads,
  * shaders/common/types.wgsl:19-63), 144 B. prev_view_proj is column-major. */

@@ -417,6 +417,190 @@ static void trace_world_one(const orc_world* w, const arc_trace_params* p, const
   if (cnt) *cnt = c;
 }
 
+/* ---- shadow any-hit walkers: trace_shadow_vis (traversal64.wgsl:571-773) and
+ * trace_shadow_vis_di (:781-989) ------------------------------------------------------------ */
+static inline int is_emissive_mat(uint32_t m) { /* :775-777 */
+  return (m >= 19u && m <= 22u) || m == 24u || m == 27u;
+}
+
+static void shadow_one(const orc_world* w, uint32_t mode, uint32_t ray_steps_w, const arc_ray* ray,
+                       float* vis, arc_shadow_rec* rec) {
+  const float o[3] = {ray->ox, ray->oy, ray->oz};
+  const float dir[3] = {ray->dx, ray->dy, ray->dz};
+  float vo[3], safe[3], inv[3], wmax[3], noi[3];
+  int pos[3];
+  uint32_t wmaxi[3];
+  for (int a = 0; a < 3; ++a) vo[a] = o[a] * 16.0f; /* :576 / :786 */
+  const float max_t_v = ray->t_max * 16.0f;
+  for (int a = 0; a < 3; ++a) {
+    safe[a] = fabsf(dir[a]) < 1e-7f ? 1e-7f : dir[a]; /* :579 */
+    inv[a] = 1.0f / safe[a];
+    wmax[a] = (float)w->grid[a] * 256.0f;
+  }
+  float tminv[3], tmaxv[3];
+  for (int a = 0; a < 3; ++a) { /* intersect_aabb :134-142 */
+    float eb = inv[a] > 0.0f ? 0.0f : wmax[a];
+    float xb = inv[a] > 0.0f ? wmax[a] : 0.0f;
+    tminv[a] = (eb - vo[a]) * inv[a];
+    tmaxv[a] = (xb - vo[a]) * inv[a];
+  }
+  const float wh_x = fmaxf(fmaxf(tminv[0], tminv[1]), tminv[2]);
+  const float wh_y = fminf(fminf(tmaxv[0], tmaxv[1]), tmaxv[2]);
+  memset(rec, 0, sizeof *rec);
+  if (wh_y < fmaxf(wh_x, 0.0f) || wh_x >= max_t_v) { /* :587-589 */
+    *vis = 1.0f;
+    rec->outcome = ARC_SHADOW_MISS_WORLD;
+    return;
+  }
+  const float t_enter = fmaxf(wh_x, 0.0f);
+  float t = t_enter;
+  const float t_limit = fminf(wh_y, max_t_v);
+  uint32_t max_steps = ray_steps_w; /* :593 */
+  if (mode == ARC_SHADOW_DI) max_steps = max_steps < 32u ? 32u : (max_steps > 48u ? 48u : max_steps); /* :803 */
+  for (int a = 0; a < 3; ++a) {
+    pos[a] = inv[a] >= 0.0f;
+    noi[a] = -vo[a] * inv[a];
+    wmaxi[a] = w->grid[a] << 8;
+  }
+  uint32_t vi[3];
+  for (int a = 0; a < 3; ++a) { /* :599-600 */
+    float sp = clampf_(vo[a] + dir[a] * (t + 1e-4f), 0.0f, wmax[a] - 1e-3f);
+    vi[a] = f2u(sp);
+  }
+  uint32_t prev[3] = {~0u, ~0u, ~0u};
+  uint32_t cached_shift = 6u;
+  arc_node64 l1 = {0, 0, 0, 0}, l2 = {0, 0, 0, 0}, l3 = {0, 0, 0, 0};
+  uint32_t cached_root = ARC_EMPTY_CHUNK_SENTINEL;
+  const uint32_t gx = w->grid[0], gy = w->grid[1];
+  uint32_t steps = 0u;
+  uint32_t step;
+  for (step = 0; step < max_steps; ++step) { /* :610 */
+    if (t >= t_limit) break;
+    if (vi[0] >= wmaxi[0] || vi[1] >= wmaxi[1] || vi[2] >= wmaxi[2]) {
+      t = t_limit;
+      break;
+    }
+    steps++;
+    const uint32_t diff = (prev[0] ^ vi[0]) | (prev[1] ^ vi[1]) | (prev[2] ^ vi[2]);
+    memcpy(prev, vi, sizeof prev);
+    uint32_t root_idx = cached_root;
+    if (diff >= 256u) { /* :624-630 */
+      cached_shift = 6u;
+      const uint32_t c_idx = (vi[0] >> 8) + (vi[1] >> 8) * gx + (vi[2] >> 8) * gx * gy;
+      root_idx = w->roots[c_idx];
+      cached_root = root_idx;
+    }
+    if (root_idx == ARC_EMPTY_CHUNK_SENTINEL) { /* :632-646 */
+      uint32_t bmin[3], bmax[3];
+      for (int a = 0; a < 3; ++a) {
+        bmin[a] = vi[a] & ~255u;
+        bmax[a] = bmin[a] + 256u;
+      }
+      step_res st = step_dda_box(vo, dir, inv, noi, pos, bmin, bmax, t);
+      t = st.t;
+      memcpy(vi, st.vi, sizeof vi);
+      continue;
+    }
+    uint32_t min_shift = 0u;
+    if (mode == ARC_SHADOW_SUN) { /* :648-654 */
+      if (t > 48.0f || step > 6u) min_shift = 2u;
+      if (t > 192.0f || step > 16u) min_shift = 4u;
+    } else { /* :858-865 */
+      const float dist_rem = t_limit - t;
+      if (t > 48.0f && dist_rem > 32.0f && step > 6u) min_shift = 2u;
+      if (t > 160.0f && dist_rem > 64.0f && step > 14u) min_shift = 4u;
+    }
+    arc_node64 node;
+    uint32_t shift;
+    if (diff < 4u && min_shift == 0u && cached_shift == 0u) { /* :656-669 */
+      node = l3;
+      shift = 0u;
+    } else if (diff < 16u && min_shift <= 2u && cached_shift <= 2u) {
+      node = l2;
+      shift = 2u;
+    } else if (diff < 64u && min_shift <= 4u && cached_shift <= 4u) {
+      node = l1;
+      shift = 4u;
+    } else {
+      node = w->nodes[root_idx];
+      shift = 6u;
+    }
+    cached_shift = shift;
+    const uint32_t l[3] = {vi[0] & 255u, vi[1] & 255u, vi[2] & 255u};
+    for (;;) { /* :672-755 / :882-971 */
+      const uint32_t b = (((l[2] >> shift) & 3u) << 4) | (((l[1] >> shift) & 3u) << 2) | ((l[0] >> shift) & 3u);
+      uint32_t span = 0u;
+      if ((node.child_ptr & ARC_IS_UNIFORM_FLAG) != 0u && ((node.lod_payload >> 24) & 0xFFu) == ARC_MAT_WATER) {
+        span = 4u << shift; /* water fast-skip */
+      } else if (!has_child(&node, b)) {
+        span = 1u << shift;
+      } else {
+        const int is_leaf = (node.child_ptr & ARC_IS_LEAF_FLAG) != 0u;
+        const int is_uniform = (node.child_ptr & ARC_IS_UNIFORM_FLAG) != 0u;
+        const int stop_lod = shift <= min_shift;
+        const int stop = mode == ARC_SHADOW_SUN ? (is_leaf || stop_lod) : (is_leaf || is_uniform || stop_lod);
+        if (stop) {
+          uint32_t mat = (node.lod_payload >> 24) & 0xFFu;
+          if (mode == ARC_SHADOW_DI && is_leaf && !is_uniform && shift == 0u)
+            mat = brick_mat(w, node.child_ptr & ARC_PTR_INDEX_MASK, b);
+          if (mat == ARC_MAT_WATER) {
+            span = 1u << shift;
+          } else {
+            int lit = 0;
+            if (mode == ARC_SHADOW_DI && is_emissive_mat(mat) && t >= t_limit - 16.0f) lit = 1; /* :960-962 */
+            *vis = lit ? 1.0f : 0.0f;
+            rec->t_voxels = t;
+            rec->steps = steps;
+            rec->vx = (uint16_t)vi[0];
+            rec->vy = (uint16_t)vi[1];
+            rec->vz = (uint16_t)vi[2];
+            rec->outcome = lit ? ARC_SHADOW_EMITTER : ARC_SHADOW_BLOCKED;
+            return;
+          }
+        }
+      }
+      if (span != 0u) {
+        uint32_t bmin[3], bmax[3];
+        for (int a = 0; a < 3; ++a) {
+          bmin[a] = vi[a] & ~(span - 1u);
+          bmax[a] = bmin[a] + span;
+        }
+        step_res st = step_dda_box(vo, dir, inv, noi, pos, bmin, bmax, t);
+        t = st.t;
+        memcpy(vi, st.vi, sizeof vi);
+        break;
+      }
+      node = w->nodes[child_offset(&node, b)];
+      shift -= 2u;
+      cached_shift = shift;
+      if (shift == 4u) l1 = node;
+      else if (shift == 2u) l2 = node;
+      else if (shift == 0u) l3 = node;
+    }
+  }
+  rec->t_voxels = t;
+  rec->steps = steps;
+  rec->vx = (uint16_t)vi[0];
+  rec->vy = (uint16_t)vi[1];
+  rec->vz = (uint16_t)vi[2];
+  const int budget = step >= max_steps;
+  if (mode == ARC_SHADOW_SUN) { /* :762-772 */
+    int dark = 0;
+    if (t < t_limit) {
+      const float q = dir[1] * t;
+      const float py = vo[1] + q;
+      if (py * 0.0625f < 32.0f) dark = 1;
+    }
+    *vis = dark ? 0.0f : 1.0f;
+    rec->outcome = dark ? ARC_SHADOW_BUDGET_DARK : (t < t_limit ? ARC_SHADOW_BUDGET_LIT : ARC_SHADOW_CLEAR);
+  } else { /* :976-984 */
+    const int lit = t >= t_limit - 2.0f;
+    *vis = lit ? 1.0f : 0.0f;
+    rec->outcome = lit ? (t >= t_limit ? ARC_SHADOW_CLEAR : ARC_SHADOW_BUDGET_LIT) : ARC_SHADOW_BUDGET_DARK;
+  }
+  (void)budget;
+}
+
 /* ---- raycast_chunk_tree: raycast_cpu.rs:28-218 ------------------------------------------ */
 /* Operates on chunk 0 of the pools (root = roots[0]); pool indices are global, which is the
  * relocated image of the chunk's local nodes (world_map.rs:705-735). */
@@ -672,6 +856,9 @@ typedef struct job {
   orc_counters* cnt;
   uint64_t next;
   pthread_mutex_t mu;
+  const arc_shadow_params* sp; /* kind 3 */
+  float* vis;
+  arc_shadow_rec* srec;
 } job;
 
 static void* worker(void* arg) {
@@ -692,8 +879,10 @@ static void* worker(void* arg) {
         const float o[3] = {j->rays[i].ox, j->rays[i].oy, j->rays[i].oz};
         const float d[3] = {j->rays[i].dx, j->rays[i].dy, j->rays[i].dz};
         trace_chunk_one(j->w, j->w->roots[0], o, d, j->rays[i].t_max, &j->out[i], c);
-      } else {
+      } else if (j->kind == 2) {
         trace_meters_one(j->w, &j->rays[i], &j->out[i]);
+      } else {
+        shadow_one(j->w, j->sp->mode, j->sp->max_steps, &j->rays[i], &j->vis[i], &j->srec[i]);
       }
     }
   }
@@ -730,6 +919,12 @@ void orc_trace_chunk(const orc_world* w, const arc_ray* rays, uint64_t n, arc_tr
 void orc_trace_meters(const orc_world* w, const arc_ray* rays, uint64_t n, arc_trace_result* out,
                       int nthreads) {
   job j = {2, w, NULL, rays, n, out, NULL, 0, PTHREAD_MUTEX_INITIALIZER};
+  run_job(&j, nthreads);
+}
+
+void orc_trace_shadow(const orc_world* w, const arc_shadow_params* sp, const arc_ray* rays, uint64_t n, float* vis,
+                      arc_shadow_rec* rec, int nthreads) {
+  job j = {3, w, NULL, rays, n, NULL, NULL, 0, PTHREAD_MUTEX_INITIALIZER, sp, vis, rec};
   run_job(&j, nthreads);
 }
 

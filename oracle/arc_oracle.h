@@ -10,6 +10,7 @@
  *   orc_trace_chunk   raycast_chunk_tree        crates/arcadia-core/src/raycast_cpu.rs:28-218
  *   orc_trace_meters  WorldMap::raycast_meters  crates/arcadia-world/src/world_map.rs:464-545
  *   orc_query_voxel   query_voxel_world_meters  shaders/common/traversal64.wgsl:182-220
+ *   orc_trace_shadow  trace_shadow_vis / _di    shaders/common/traversal64.wgsl:571-773 / 781-989
  * plus an independent checker that shares no code with the above:
  *   orc_brute_dense   Amanatides-Woo unit-voxel walk in double precision over a dense grid.
  *
@@ -59,6 +60,10 @@ void orc_trace_chunk(const orc_world* w, const arc_ray* rays, uint64_t n, arc_tr
                      orc_counters* counters, int nthreads);
 void orc_trace_meters(const orc_world* w, const arc_ray* rays, uint64_t n, arc_trace_result* out,
                       int nthreads);
+/* Shadow any-hit walkers (mode ARC_SHADOW_SUN / ARC_SHADOW_DI): visibility and the per-ray
+ * record the kernel writes. */
+void orc_trace_shadow(const orc_world* w, const arc_shadow_params* sp, const arc_ray* rays, uint64_t n, float* vis,
+                      arc_shadow_rec* rec, int nthreads);
 /* Compact records as the kernels write them. */
 void orc_compact(const arc_trace_result* full, uint64_t n, int world_units, arc_hit* out);
 

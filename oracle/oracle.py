@@ -26,6 +26,8 @@ RESULT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("norm
                          ("reserved", "<u4", (2,))])
 COUNTER_DTYPE = np.dtype([("node_loads", "<u4"), ("brick_loads", "<u4"), ("root_loads", "<u4"), ("steps", "<u4"),
                           ("level_loads", "<u4", (4,)), ("restarts", "<u4", (4,)), ("empty_chunk_steps", "<u4")])
+SHADOW_REC_DTYPE = np.dtype([("t_voxels", "<f4"), ("steps", "<u4"), ("vx", "<u2"), ("vy", "<u2"), ("vz", "<u2"),
+                             ("outcome", "u1"), ("reserved", "u1")])
 BRUTE_DTYPE = np.dtype([("t", "<f8"), ("vx", "<i4"), ("vy", "<i4"), ("vz", "<i4"), ("hit", "<i4"), ("face", "<i4"),
                         ("tie", "<i4"), ("mat", "<i4"), ("pad", "<i4")])
 
@@ -74,6 +76,8 @@ def lib() -> C.CDLL:
         h.orc_sharc_query.argtypes = [C.c_uint32, P, P, C.c_uint64, P, P]
         for f in (h.orc_sharc_make_key, h.orc_sharc_deposit, h.orc_sharc_resolve, h.orc_sharc_query):
             f.restype = None
+        h.orc_trace_shadow.argtypes = [P, P, P, C.c_uint64, P, P, C.c_int]
+        h.orc_trace_shadow.restype = None
         h.orc_f32_to_f16.argtypes = [C.c_float]
         h.orc_f32_to_f16.restype = C.c_uint16
         for f in (h.orc_trace_world, h.orc_trace_chunk, h.orc_trace_meters, h.orc_compact, h.orc_decode_dense,
@@ -131,6 +135,22 @@ def trace_chunk(world: World, rays: np.ndarray, nthreads: int = 0):
         lib().orc_trace_chunk(world.ref, rays.ctypes.data, len(rays), out.ctypes.data, cnt.ctypes.data,
                               _threads(nthreads))
     return out, cnt
+
+
+class ShadowParams(C.Structure):  # layout of arc_shadow_params
+    _fields_ = [("mode", C.c_uint32), ("max_steps", C.c_uint32)]
+
+
+def trace_shadow(world: World, mode: int, max_steps: int, rays: np.ndarray, nthreads: int = 0):
+    """trace_shadow_vis (mode 0) / trace_shadow_vis_di (mode 1) -> (visibility f32, records)."""
+    rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+    vis = np.zeros(len(rays), np.float32)
+    rec = np.zeros(len(rays), SHADOW_REC_DTYPE)
+    sp = ShadowParams(mode, max_steps)
+    if len(rays):
+        lib().orc_trace_shadow(world.ref, C.byref(sp), rays.ctypes.data, len(rays), vis.ctypes.data, rec.ctypes.data,
+                               _threads(nthreads))
+    return vis, rec
 
 
 def trace_meters(world: World, rays: np.ndarray, nthreads: int = 0):
