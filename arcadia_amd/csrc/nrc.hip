@@ -26,6 +26,8 @@
 #include <new>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "../../include/arcadia_hip.h"
 
 namespace {
@@ -210,10 +212,38 @@ __device__ __forceinline__ floatx4 forward_tile(const MlpLds& L, const half4 xb[
   return y;
 }
 
+// Spatial sort key of a query: Morton order of its normalized position on a 256^3 lattice (the
+// order inference visits the queries in, so the lanes of a tile gather neighbouring table
+// entries); queries without an x1 hit sort last.
+constexpr int kSortBits = 24;
+
+__device__ __forceinline__ uint32_t spread3_8(uint32_t v) {  // bit i -> bit 3i, 8 bits
+  v = (v | (v << 8)) & 0x0F00Fu;
+  v = (v | (v << 4)) & 0x0C30C3u;
+  v = (v | (v << 2)) & 0x249249u;
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_nrc_keys(GridInfo g, const arc_path_sample* __restrict__ s, uint32_t n,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SampleIn si = load_sample(s, i, n, g);
+  uint32_t k = (1u << kSortBits) - 1u;
+  if (si.valid) {
+    const uint32_t x = (uint32_t)(si.p[0] * 256.0f), y = (uint32_t)(si.p[1] * 256.0f), z = (uint32_t)(si.p[2] * 256.0f);
+    k = spread3_8(x) | (spread3_8(y) << 1) | (spread3_8(z) << 2);
+  }
+  keys[i] = k;
+  idx[i] = i;
+}
+
+// perm (may be null): tile lane j handles query perm[16 * tile + j]; each query's output depends
+// on that query alone, so the visiting order changes no result.
 __global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g, const uint32_t* __restrict__ table_h,
                                                    const _Float16* __restrict__ mlp_h,
                                                    const arc_path_sample* __restrict__ s, uint32_t n,
-                                                   float4* __restrict__ out) {
+                                                   const uint32_t* __restrict__ perm, float4* __restrict__ out) {
   __shared__ MlpLds L;
   stage(mlp_h + kOffW1, kHid, kIn, L.w1, kPadIn, nullptr, 0);
   stage(mlp_h + kOffW2, kHid, kHid, L.w2, kPadHid, nullptr, 0);
@@ -224,12 +254,13 @@ __global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g, const uint32_t* _
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
   for (uint32_t tile = wave; tile * 16u < n; tile += nwaves) {
-    const uint32_t i = tile * 16u + (lane & 15u);
+    const uint32_t j = tile * 16u + (lane & 15u);
+    const uint32_t i = perm && j < n ? perm[j] : j;
     const SampleIn si = load_sample(s, i, n, g);
     half4 xb[3], a1[4], a2[4], a3[4];
     input_tiles(g, table_h, si, grp, xb);
     const floatx4 y = forward_tile(L, xb, lane, a1, a2, a3);
-    if (grp == 0 && i < n) out[i] = make_float4(y[0], y[1], y[2], si.valid ? 1.0f : 0.0f);
+    if (grp == 0 && j < n) out[i] = make_float4(y[0], y[1], y[2], si.valid ? 1.0f : 0.0f);
   }
 }
 
@@ -422,6 +453,11 @@ struct arc_nrc {
   float* loss = nullptr;
   uint64_t step = 0;
   uint32_t n_cu = 0;
+  // query sort workspace: keys in/out, indices in/out (sort_cap entries each) + radix-sort scratch
+  uint32_t* sort_buf = nullptr;
+  uint64_t sort_cap = 0;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
 };
 
 namespace {
@@ -443,8 +479,44 @@ int nrc_alloc(arc_nrc* r) {
 
 void nrc_free(arc_nrc* r) {
   void* ps[] = {r->grid_p, r->grid_m, r->grid_v, r->grid_g, r->grid_h, r->mlp_p, r->mlp_m,
-                r->mlp_v,  r->mlp_g, r->mlp_h, r->act,    r->loss};
+                r->mlp_v,  r->mlp_g, r->mlp_h, r->act,    r->loss,   r->sort_buf, r->sort_tmp};
   for (void* p : ps) (void)hipFree(p);
+}
+
+// Sorts the queries of an inference launch by their Morton key (k_nrc_keys) into the object's
+// workspace (grown on demand) and returns the visiting order: the lanes of a tile then gather
+// neighbouring table entries (measured on MI355X at 1080p: 0.84 -> 0.52 ms of inference for
+// 0.1 ms of sorting). Calls on one cache object share the workspace: issue them on one stream.
+int sort_queries(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, hipStream_t s, const uint32_t** perm) {
+  if (n > r->sort_cap) {
+    size_t tmp = 0;
+    if (rocprim::radix_sort_pairs(nullptr, tmp, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, (size_t)n, 0, kSortBits, s) != hipSuccess)
+      return ARC_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return ARC_ERR_HIP;
+    (void)hipFree(r->sort_buf);
+    (void)hipFree(r->sort_tmp);
+    r->sort_buf = nullptr;
+    r->sort_tmp = nullptr;
+    r->sort_cap = 0;
+    if (hipMalloc(&r->sort_buf, n * 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&r->sort_tmp, tmp > 0 ? tmp : 1) != hipSuccess)
+      return ARC_ERR_HIP;
+    r->sort_cap = n;
+    r->sort_tmp_bytes = tmp;
+  }
+  uint32_t* keys_in = r->sort_buf;
+  uint32_t* keys_out = keys_in + r->sort_cap;
+  uint32_t* idx_in = keys_out + r->sort_cap;
+  uint32_t* idx_out = idx_in + r->sort_cap;
+  hipLaunchKernelGGL(k_nrc_keys, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, r->g, d_samples, (uint32_t)n,
+                     keys_in, idx_in);
+  size_t tmp = r->sort_tmp_bytes;
+  if (rocprim::radix_sort_pairs(r->sort_tmp, tmp, keys_in, keys_out, idx_in, idx_out, (size_t)n, 0, kSortBits, s) !=
+      hipSuccess)
+    return ARC_ERR_HIP;
+  *perm = idx_out;
+  return hipGetLastError() == hipSuccess ? ARC_OK : ARC_ERR_HIP;
 }
 
 int launch_zero(float* p, uint64_t n, hipStream_t s) {
@@ -542,11 +614,13 @@ int arc_nrc_infer(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, floa
   if (!r || (n && (!d_samples || !d_out)) || n > 0x7FFFFFFFull) return ARC_ERR_INVALID;
   if (n == 0) return ARC_OK;
   if (hipSetDevice(r->device) != hipSuccess) return ARC_ERR_HIP;
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t* perm = nullptr;
+  if (sort_queries(r, d_samples, n, s, &perm) != ARC_OK) return ARC_ERR_HIP;
   const uint64_t tiles = (n + 15) / 16;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, (uint64_t)r->n_cu * 8);
-  hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, (hipStream_t)stream, r->g,
-                     reinterpret_cast<const uint32_t*>(r->grid_h), r->mlp_h, d_samples, (uint32_t)n,
-                     reinterpret_cast<float4*>(d_out));
+  hipLaunchKernelGGL(k_nrc_infer, dim3(blocks), dim3(256), 0, s, r->g, reinterpret_cast<const uint32_t*>(r->grid_h),
+                     r->mlp_h, d_samples, (uint32_t)n, perm, reinterpret_cast<float4*>(d_out));
   return hipGetLastError() == hipSuccess ? ARC_OK : ARC_ERR_HIP;
 }
 
@@ -568,6 +642,8 @@ int arc_nrc_backward(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, v
   }
   const uint64_t tiles = ld / 16;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, (uint64_t)r->n_cu * 4);
+  // Training keeps the batch order: in Morton order the lanes of a tile hit the same coarse-level
+  // entries and their gradient atomics serialize (measured slower at 16K and 66K samples).
   hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(256), 0, s, r->g, reinterpret_cast<const uint32_t*>(r->grid_h),
                      r->mlp_h, d_samples, (uint32_t)n, r->act_ld, r->act, r->grid_g, r->loss);
   const uint32_t ldv = r->act_ld;

@@ -2,7 +2,7 @@
 traversal generates, judged against an FP64-accumulated path-traced reference.
 
 A frame:
-  1. training paths: one 4-bounce path per pixel of a ~2^16-pixel view of the camera
+  1. training paths: one 4-bounce path per pixel of a ~2^14-pixel view of the camera
      (arc_batch_trace, one persistent launch), shaded into path samples (arc_batch_shade);
   2. SHARC: deposit each sample's radiance beyond x1 at x1's surfel, resolve;
      neural cache: one Adam step on the samples;
@@ -49,7 +49,7 @@ def add_emissive_clusters(world: N.HostWorld, grid, seed: int = 1, count: int = 
 class CacheRun:
     """Buffers and launches of the workload on one camera."""
 
-    def __init__(self, ctx: N.Context, cam: N.Camera, width: int, height: int, train_size=(341, 192),
+    def __init__(self, ctx: N.Context, cam: N.Camera, width: int, height: int, train_size=(171, 96),
                  shade: N.ShadeParams | None = None, sharc_log2: int = 20, nrc: N.NrcParams | None = None,
                  max_dist_m: float = 250.0, seed: int = 1):
         import torch

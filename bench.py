@@ -241,7 +241,7 @@ def cache_leg(args, threads):
     ctx = A.Context(0)
     ctx.upload(*pools)
     cam = A.camera_look((31.3, 58.0, 27.1), 0.7, -0.5, 65.0, 1920, 1080)
-    run = LC.CacheRun(ctx, cam, 1920, 1080, train_size=(341, 192), sharc_log2=20,
+    run = LC.CacheRun(ctx, cam, 1920, 1080, train_size=(171, 96), sharc_log2=20,
                       nrc=A.NrcParams.default(bbox_size=64.0, log2_table=19), seed=args.seed)
     print(f"[bench] cache: scene {time.time() - t0:.1f}s, {run.n_eval} eval slots, {run.n_train} training paths "
           f"per frame", file=sys.stderr, flush=True)

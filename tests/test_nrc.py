@@ -104,6 +104,15 @@ def test_gpu_infer_and_gradients_match_restatement(ctx):
     print(f"[nrc] |y| max {np.abs(y).max():.3f}", flush=True)
     assert np.allclose(got[:, :3], y, rtol=2e-3, atol=2e-3), np.abs(got[:, :3] - y).max()
     assert np.array_equal(got[:, 3], ((s["flags"] & 4) != 0).astype(np.float32))
+    # inference visits the queries in Morton order of their positions: a permuted batch (and a
+    # second, smaller call reusing the sort workspace) gives the permuted outputs bit for bit
+    perm = rng.permutation(n)
+    for m in (n, n // 3):
+        dp = torch.from_numpy(s[perm[:m]].view(np.uint8).copy()).cuda()
+        outp = torch.zeros(m * 4, dtype=torch.float32, device="cuda")
+        nrc.infer(dp.data_ptr(), m, outp.data_ptr())
+        torch.cuda.synchronize()
+        assert np.array_equal(outp.cpu().numpy().reshape(m, 4).view(np.uint32), got[perm[:m]].view(np.uint32))
     nrc.zero_grads()
     nrc.loss(reset=True)
     nrc.backward(d.data_ptr(), n)
