@@ -43,6 +43,8 @@ struct arc_ctx {
   arc_ray* d_gb_rays = nullptr;
   arc_trace_result* d_gb_res = nullptr;
   uint64_t gb_n = 0;
+  // rigid-body instances of the last arc_trace_rigid_device call (ARC_MAX_RIGID_BODIES slots)
+  arc_rigid_body* d_bodies = nullptr;
   // bumped whenever the device pool addresses change (upload, pool growth): captured graphs bake them
   uint64_t world_gen = 0;
   // arc_world_patch staging (pinned host + device copy of the same size)
@@ -188,6 +190,7 @@ void arc_destroy(arc_ctx* c) {
   (void)hipFree(c->d_roots);
   (void)hipFree(c->d_nodes);
   (void)hipFree(c->d_bricks);
+  (void)hipFree(c->d_bodies);
   (void)hipFree(c->d_queue);
   (void)hipFree(c->d_rays);
   (void)hipFree(c->d_hits);
@@ -459,6 +462,56 @@ int arc_trace_shadow_device(arc_ctx* c, const arc_shadow_params* sp, const arc_r
   a.batch = cfg.batch_rays;
   hipError_t e = arcdev::launch_shadow(a, (hipStream_t)stream);
   if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "shadow launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
+int arc_trace_rigid_device(arc_ctx* c, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* d_rays,
+                           uint64_t n, arc_rigid_hit* d_out, void* stream) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "trace before arc_world_upload");
+  if (n == 0) return ARC_OK;
+  if (!d_rays || !d_out || (n_bodies && !bodies)) return fail(c, ARC_ERR_INVALID, "null ray, output or body pointer");
+  if (n > 0x7FFFFFFFull) return fail(c, ARC_ERR_INVALID, "at most 2^31-1 rays per launch");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const uint32_t nb = std::min<uint32_t>(n_bodies, ARC_MAX_RIGID_BODIES);  // trace_rigid_bodies :1219
+  hipStream_t s = (hipStream_t)stream;
+  if (nb) {
+    if (!c->d_bodies) HIP_TRY(c, hipMalloc(&c->d_bodies, ARC_MAX_RIGID_BODIES * sizeof(arc_rigid_body)));
+    HIP_TRY(c, hipMemcpyAsync(c->d_bodies, bodies, nb * sizeof(arc_rigid_body), hipMemcpyHostToDevice, s));
+  }
+  arcdev::RigidLaunchArgs a{};
+  a.nodes = c->d_nodes;
+  a.bricks = c->d_bricks;
+  a.n_nodes = c->n_nodes;
+  a.n_bricks = c->n_bricks;
+  a.bodies = c->d_bodies;
+  a.n_bodies = nb;
+  a.rays = d_rays;
+  a.n = n;
+  a.out = d_out;
+  hipError_t e = arcdev::launch_rigid(a, s);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "rigid launch: %s", hipGetErrorString(e));
+  return ARC_OK;
+}
+
+int arc_query_voxels_device(arc_ctx* c, const float* d_points_m, uint64_t n, uint32_t* d_mat, void* stream) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "query before arc_world_upload");
+  if (n == 0) return ARC_OK;
+  if (!d_points_m || !d_mat) return fail(c, ARC_ERR_INVALID, "null point or output pointer");
+  if (n > 0x7FFFFFFFull) return fail(c, ARC_ERR_INVALID, "at most 2^31-1 points per launch");
+  arcdev::QueryLaunchArgs a{};
+  std::memcpy(a.grid, c->grid, sizeof a.grid);
+  a.roots = c->d_roots;
+  a.nodes = c->d_nodes;
+  a.bricks = c->d_bricks;
+  a.n_nodes = c->n_nodes;
+  a.n_bricks = c->n_bricks;
+  a.points = d_points_m;
+  a.n = n;
+  a.mat = d_mat;
+  hipError_t e = arcdev::launch_query(a, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "query launch: %s", hipGetErrorString(e));
   return ARC_OK;
 }
 

@@ -69,6 +69,32 @@ struct ShadowLaunchArgs {
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_shadow(const ShadowLaunchArgs& a, hipStream_t stream);
+
+// One launch of the rigid-body walker: `bodies` is a device copy of at most 32 instances.
+struct RigidLaunchArgs {
+  const arc_node64* nodes;
+  const arc_brick64* bricks;
+  uint64_t n_nodes, n_bricks;
+  const arc_rigid_body* bodies;
+  uint32_t n_bodies;
+  const arc_ray* rays;
+  uint64_t n;
+  arc_rigid_hit* out;
+};
+hipError_t launch_rigid(const RigidLaunchArgs& a, hipStream_t stream);
+
+// One launch of the point query.
+struct QueryLaunchArgs {
+  uint32_t grid[3];
+  const uint32_t* roots;
+  const arc_node64* nodes;
+  const arc_brick64* bricks;
+  uint64_t n_nodes, n_bricks;
+  const float* points;
+  uint64_t n;
+  uint32_t* mat;
+};
+hipError_t launch_query(const QueryLaunchArgs& a, hipStream_t stream);
 hipError_t shadow_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu);
 hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu);
 // Resident workgroups per CU of the traversal kernel at `block_threads`.

@@ -59,6 +59,15 @@ SHADOW_REC_DTYPE = np.dtype([("t_voxels", "<f4"), ("steps", "<u4"), ("vx", "<u2"
                              ("outcome", "u1"), ("reserved", "u1")])
 
 
+RIGID_BODY_DTYPE = np.dtype([("world_to_local", "<f4", (16,)), ("local_to_world", "<f4", (16,)),
+                             ("prev_local_to_world", "<f4", (16,)), ("local_bounds_and_root", "<f4", (4,)),
+                             ("world_sphere", "<f4", (4,))])
+ARC_MAX_RIGID_BODIES = 32  # trace_rigid_bodies reads min(count, 32) instances
+RIGID_HIT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("prev_hit_pos_m", "<f4", (3,)),
+                            ("normal", "<f4", (3,)), ("contact_ao", "<f4"), ("mat", "<u4"), ("voxel_hash", "<u4"),
+                            ("hit", "<u4"), ("body", "<u4"), ("reserved", "<u4")])
+
+
 class ShadowParams(C.Structure):
     """arc_shadow_params: mode ARC_SHADOW_SUN (trace_shadow_vis) / ARC_SHADOW_DI (trace_shadow_vis_di),
     max_steps = the quality's shadow_max_steps (72 at high quality)."""
@@ -208,6 +217,9 @@ _SIGS = [
     ("arc_world_read_pools", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p,
                                        C.c_uint64, C.c_uint64]),
     ("arc_trace", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("arc_trace_rigid_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64, C.c_void_p,
+                                         C.c_void_p]),
+    ("arc_query_voxels_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     ("arc_trace_shadow_device", C.c_int, [C.c_void_p, C.POINTER(ShadowParams), C.c_void_p, C.c_uint64, C.c_void_p,
                                           C.c_void_p, C.c_void_p]),
     ("arc_trace_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p,
@@ -728,6 +740,18 @@ class Context:
         sp = ShadowParams(mode, max_steps)
         self._check(lib().arc_trace_shadow_device(self._h, C.byref(sp), d_rays, n, d_vis, d_rec or None,
                                                   stream or None), "arc_trace_shadow_device")
+
+    def trace_rigid_device(self, bodies: np.ndarray, d_rays: int, n: int, d_out: int, stream: int = 0) -> None:
+        """trace_rigid_bodies over device rays; `bodies` is a host RIGID_BODY_DTYPE array (<= 32 used),
+        d_out receives RIGID_HIT_DTYPE records."""
+        b = np.ascontiguousarray(bodies, RIGID_BODY_DTYPE)
+        self._check(lib().arc_trace_rigid_device(self._h, b.ctypes.data if len(b) else None, len(b), d_rays, n, d_out,
+                                                 stream or None), "arc_trace_rigid_device")
+
+    def query_voxels_device(self, d_points: int, n: int, d_mat: int, stream: int = 0) -> None:
+        """query_voxel_world_meters at n device points (3 f32 metres each) -> u32 materials."""
+        self._check(lib().arc_query_voxels_device(self._h, d_points, n, d_mat, stream or None),
+                    "arc_query_voxels_device")
 
     def trace_full_device(self, params: TraceParams, d_rays: int, n: int, d_out: int, stream: int = 0) -> None:
         self._check(lib().arc_trace_full_device(self._h, C.byref(params), d_rays, n, d_out, stream or None),

@@ -225,6 +225,88 @@ def shadow_leg(args, ctx, pools, prays, phits, stream, threads):
     return out
 
 
+def rigid_leg(args, ctx, pools, prays, phits, stream, threads):
+    """Rigid voxel bodies over the full-frame world, as the primary G-buffer pass traces them
+    (primary_gbuffer.wgsl:123-125): every 1080p primary ray walks the 32 bodies (the instance
+    limit) up to its world hit distance (250 m without one). The bodies are 32^3-voxel trees of
+    three shapes (a solid rock, a hollow crate, scattered debris) at random orientations, each
+    floating 1.5 m above a random visible surface point 8..48 m from the camera. The device
+    records are checked against the oracle on a sample. The world pools are re-uploaded with the
+    body trees appended."""
+    import torch
+
+    import arcadia_amd as A
+    from arcadia_amd import rigid as RB
+    from arcadia_amd import shadow_rays as S
+
+    rng = np.random.default_rng(args.seed + 7)
+    g = np.indices((32, 32, 32)).reshape(3, -1).T
+    r2 = ((g - 15.5) ** 2).sum(1)
+    shell = np.any((g == 1) | (g == 30), axis=1) & np.all((g >= 1) & (g <= 30), axis=1)
+    shapes = [g[r2 < 14.0 ** 2], g[shell], rng.integers(0, 32, (3000, 3))]
+    bpools, roots = pools, []
+    for k, vox in enumerate(shapes):
+        w = A.HostWorld((1, 1, 1))
+        mats = rng.integers(1, 16, (len(vox), 1))
+        w.set_voxels(np.ascontiguousarray(np.concatenate([vox, mats], 1), np.int32))
+        w.pack()
+        bpools, root = RB.append_body_tree(bpools, w.pools())
+        roots.append(root)
+    hidx, hp, hn = S.hit_points(prays, phits)
+    dist = phits["t_voxels"][hidx] / 16.0
+    pick = rng.choice(np.nonzero((dist > 8.0) & (dist < 48.0))[0], A.ARC_MAX_RIGID_BODIES, replace=False)
+    bodies = []
+    for i, j in enumerate(pick):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        a, b, c, d = q
+        rot = np.array([[1 - 2 * (c * c + d * d), 2 * (b * c - d * a), 2 * (b * d + c * a)],
+                        [2 * (b * c + d * a), 1 - 2 * (b * b + d * d), 2 * (c * d - b * a)],
+                        [2 * (b * d - c * a), 2 * (c * d + b * a), 1 - 2 * (b * b + c * c)]])
+        bodies.append(RB.body_instance(roots[i % 3], (32, 32, 32), rot, hp[j] + hn[j] * 1.5))
+    bodies = np.array(bodies)
+    rays = prays.copy()
+    hit = (phits["flags"] & 1) != 0
+    rays["t_max"] = np.where(hit, phits["t_voxels"] * np.float32(1.0 / 16.0), rays["t_max"]).astype(np.float32)
+    ctx.upload(*bpools)
+    n = len(rays)
+    sptr = stream.cuda_stream
+    d_r = torch.from_numpy(rays.view(np.uint8)).cuda()
+    d_o = torch.empty(n * 64, dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        ctx.trace_rigid_device(bodies, d_r.data_ptr(), n, d_o.data_ptr(), sptr)
+    k = max(3, min(args.steps, 10))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(k):
+        ctx.trace_rigid_device(bodies, d_r.data_ptr(), n, d_o.data_ptr(), sptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / k
+    out_h = d_o.cpu().numpy().view(A.RIGID_HIT_DTYPE)
+    sample = np.arange(0, n, max(1, n // 20000))
+    exact = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        want = O.trace_rigid(O.World(*bpools), bodies, rays[sample])
+        exact = bool(np.array_equal(want.view(np.uint32), out_h[sample].view(np.uint32)))
+    res = {
+        "metric": "Grays/s rigid-body walker (32 bodies of 32^3 voxels, 1080p primary rays cut at the world hit)",
+        "rays": int(n),
+        "bodies": int(len(bodies)),
+        "ms": round(ms, 4),
+        "grays": round(n / (ms * 1e-3) / 1e9, 4),
+        "hit_fraction": round(float((out_h["hit"] == 1).mean()), 4),
+        "oracle_sample_exact": exact,
+        "sample_rays": int(len(sample)),
+    }
+    print(f"[bench] rigid: {n} rays x {len(bodies)} bodies {ms:.3f} ms -> {res['grays']} Grays/s, "
+          f"hits {res['hit_fraction']}, exact sample {exact}", file=sys.stderr, flush=True)
+    return res
+
+
 def cache_leg(args, threads):
     """Learned radiance cache vs SHARC (2^22 entries) vs no cache, rel-MSE against the FP64
     1024-spp path-traced judge at 1920x1080 on the full-frame world with 8 emissive clusters."""
@@ -401,6 +483,7 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shapes")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-walker workload")
+    ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-body workload")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache workload")
     ap.add_argument("--cache-ref-spp", type=int, default=1024)
     ap.add_argument("--cache-frames", type=int, default=60)
@@ -417,6 +500,7 @@ def main() -> int:
         args.steps, args.warmup, args.no_cpu_baseline = min(args.steps, 5), 1, True
         args.no_batched = True
         args.no_shadow = True
+        args.no_rigid = True
         args.no_cache = True
         args.no_destruction = True
 
@@ -547,6 +631,8 @@ def main() -> int:
         out["cpu_baseline"] = cpu
     if not args.no_shadow:
         out["shadow"] = shadow_leg(args, ctx, pools, prays, phits, stream, threads)
+    if not args.no_rigid:
+        out["rigid"] = rigid_leg(args, ctx, pools, prays, phits, stream, threads)
     ctx.close()
     del d_prays, d_phits, d_brays, d_bhits
     if not args.no_batched:

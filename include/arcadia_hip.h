@@ -230,6 +230,45 @@ typedef struct arc_shadow_rec {
 int arc_trace_shadow_device(arc_ctx* ctx, const arc_shadow_params* params, const arc_ray* d_rays, uint64_t n,
                             float* d_vis, arc_shadow_rec* d_rec, void* stream);
 
+/* ---- rigid voxel bodies and point queries ------------------------------------------------- */
+/* RigidBodyInstance (shaders/common/types.wgsl:219-226; GpuRigidBodyInstance,
+ * crates/arcadia-physics/src/rigid_voxel/types.rs:20-35 and transform.rs:80-112), 224 B. The
+ * matrices are column-major 4x4 (glam to_cols_array_2d): m[4 * col + row]. The body's 64-tree
+ * lives in the node/brick pools of the context like the chunks' trees. */
+typedef struct arc_rigid_body {
+  float world_to_local[16];        /* world metres -> body-local tree metres */
+  float local_to_world[16];
+  float prev_local_to_world[16];   /* previous frame's, for the motion of the hit point */
+  float local_bounds_and_root[4];  /* local box size in metres (x, y, z); w = bits of the root node index */
+  float world_sphere[4];           /* bounding sphere: centre (m), radius (m) */
+} arc_rigid_body;
+/* RigidTraceResult (traversal64.wgsl:994-1003) plus the index of the body hit. 64 B. */
+typedef struct arc_rigid_hit {
+  float t_voxels;          /* hit body's local voxel units; max_dist * 16 without a hit */
+  float hit_pos_m[3];      /* world metres; origin + dir * max_dist without a hit */
+  float prev_hit_pos_m[3]; /* the hit point moved with prev_local_to_world */
+  float normal[3];         /* world space, normalized; (0, 1, 0) without a hit */
+  float contact_ao;        /* eval_leaf_bitwise_ao at voxel-level hits, else 1 */
+  uint32_t mat;            /* material, ARC_MAT_AIR without a hit */
+  uint32_t voxel_hash;     /* pcg_hash of the hit body's local voxel */
+  uint32_t hit;            /* 1 when a body was hit */
+  uint32_t body;           /* index of the body hit, 0xFFFFFFFF without (no reference counterpart) */
+  uint32_t reserved;
+} arc_rigid_hit;
+#define ARC_MAX_RIGID_BODIES 32u /* trace_rigid_bodies reads min(count, 32) instances (:1219) */
+/* trace_rigid_bodies (traversal64.wgsl:1204-1244) over the first min(n_bodies, 32) of `bodies`
+ * (a host array, copied to the device with the launch): for each ray (origin_m, dir, t_max =
+ * max_dist_m; t_start unused) the nearest hit over the bodies, each traced by
+ * trace_single_rigid_body (:1005-1202). Device rays and output, asynchronous on `stream`.
+ * Node and brick indices are checked against the pool sizes; a body whose tree leaves the pools
+ * traces as empty. */
+int arc_trace_rigid_device(arc_ctx* ctx, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* d_rays,
+                           uint64_t n, arc_rigid_hit* d_out, void* stream);
+/* query_voxel_world_meters (traversal64.wgsl:182-220): the material at each point (x, y, z
+ * metres, 12 B per point), ARC_MAT_AIR outside the world or in empty space. Device pointers,
+ * asynchronous on `stream`. */
+int arc_query_voxels_device(arc_ctx* ctx, const float* d_points_m, uint64_t n, uint32_t* d_mat, void* stream);
+
 /* ---- primary G-buffer pass -------------------------------------------------------------- */
 /* Camera and frame inputs of primary_raymarch (the CameraUniforms fields it reads,
  * shaders/common/types.wgsl:19-63), 144 B. prev_view_proj is column-major. */

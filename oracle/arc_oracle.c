@@ -601,6 +601,233 @@ static void shadow_one(const orc_world* w, uint32_t mode, uint32_t ray_steps_w, 
   (void)budget;
 }
 
+/* ---- rigid bodies: trace_single_rigid_body / trace_rigid_bodies (traversal64.wgsl:1005-1244)
+ * Matrix-vector products are evaluated column by column, ((c0 x + c1 y) + c2 z) + c3 w, and dot
+ * products x, y, z left to right; normalize(v) = v / sqrt(dot(v, v)); sign(0) = 0. ----------- */
+static void mat_vec(const float m[16], const float v[4], float out[3]) {
+  for (int r = 0; r < 3; ++r) {
+    float a = m[0 + r] * v[0];
+    float b = m[4 + r] * v[1];
+    float c = m[8 + r] * v[2];
+    float d = m[12 + r] * v[3];
+    out[r] = ((a + b) + c) + d;
+  }
+}
+
+static inline float dot3(const float a[3], const float b[3]) {
+  float x = a[0] * b[0], y = a[1] * b[1], z = a[2] * b[2];
+  return (x + y) + z;
+}
+
+static inline float sign_wgsl(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+static void rigid_miss(const float o[3], const float d[3], float max_dist_m, arc_rigid_hit* res) {
+  memset(res, 0, sizeof *res);
+  res->t_voxels = max_dist_m * 16.0f;
+  for (int a = 0; a < 3; ++a) {
+    const float q = d[a] * max_dist_m;
+    res->hit_pos_m[a] = o[a] + q;
+    res->prev_hit_pos_m[a] = res->hit_pos_m[a];
+  }
+  res->normal[1] = 1.0f;
+  res->mat = ARC_MAT_AIR;
+  res->contact_ao = 1.0f;
+  res->body = 0xFFFFFFFFu;
+}
+
+static void rigid_single(const orc_world* w, const arc_rigid_body* rb, uint32_t rb_idx, const float o[3],
+                         const float d[3], float max_dist_m, arc_rigid_hit* res) {
+  rigid_miss(o, d, max_dist_m, res);
+  uint32_t root_idx;
+  memcpy(&root_idx, &rb->local_bounds_and_root[3], 4);
+  if (root_idx == ARC_EMPTY_CHUNK_SENTINEL || root_idx >= w->n_nodes) return;
+  /* 1. sphere pre-reject (:1028-1045) */
+  float oc[3];
+  for (int a = 0; a < 3; ++a) oc[a] = o[a] - rb->world_sphere[a];
+  const float b_proj = dot3(oc, d);
+  const float r2 = rb->world_sphere[3] * rb->world_sphere[3];
+  const float c_dist = dot3(oc, oc) - r2;
+  if (c_dist > 0.0f && b_proj > 0.0f) return;
+  const float disc = b_proj * b_proj - c_dist;
+  if (disc < 0.0f) return;
+  const float near_m = fmaxf(-b_proj - sqrtf(disc), 0.0f);
+  if (near_m >= max_dist_m) return;
+  /* 2. local frame (:1047-1054) */
+  const float o4[4] = {o[0], o[1], o[2], 1.0f}, d4[4] = {d[0], d[1], d[2], 0.0f};
+  float lo[3], ld[3];
+  mat_vec(rb->world_to_local, o4, lo);
+  mat_vec(rb->world_to_local, d4, ld);
+  float vo[3], lmax[3], safe[3], inv[3], noi[3];
+  int pos[3];
+  for (int a = 0; a < 3; ++a) {
+    vo[a] = lo[a] * 16.0f;
+    lmax[a] = rb->local_bounds_and_root[a] * 16.0f;
+  }
+  const float max_t_v = max_dist_m * 16.0f;
+  for (int a = 0; a < 3; ++a) {
+    safe[a] = fabsf(ld[a]) < 1e-7f ? 1e-7f : ld[a];
+    inv[a] = 1.0f / safe[a];
+  }
+  float tminv[3], tmaxv[3];
+  for (int a = 0; a < 3; ++a) { /* intersect_aabb :134-142 */
+    const float eb = inv[a] > 0.0f ? 0.0f : lmax[a];
+    const float xb = inv[a] > 0.0f ? lmax[a] : 0.0f;
+    tminv[a] = (eb - vo[a]) * inv[a];
+    tmaxv[a] = (xb - vo[a]) * inv[a];
+  }
+  const float bx = fmaxf(fmaxf(tminv[0], tminv[1]), tminv[2]);
+  const float by = fminf(fminf(tmaxv[0], tmaxv[1]), tmaxv[2]);
+  const float t_enter = fmaxf(bx, 0.0f);
+  if (by < t_enter || t_enter >= max_t_v) return;
+  float t = t_enter;
+  const float t_limit = fminf(by, max_t_v);
+  uint32_t lmaxi[3];
+  for (int a = 0; a < 3; ++a) {
+    pos[a] = inv[a] >= 0.0f;
+    noi[a] = -vo[a] * inv[a];
+    lmaxi[a] = f2u(lmax[a] + 0.5f);
+  }
+  uint32_t vi[3];
+  for (int a = 0; a < 3; ++a) vi[a] = f2u(clampf_(vo[a] + ld[a] * (t + 1e-4f), 0.0f, lmax[a] - 1e-3f));
+  uint32_t prev[3] = {~0u, ~0u, ~0u};
+  uint32_t cached_shift = 6u;
+  arc_node64 l1 = {0, 0, 0, 0}, l2 = {0, 0, 0, 0}, l3 = {0, 0, 0, 0};
+  for (uint32_t step = 0; step < 64u; ++step) { /* :1077 */
+    if (t >= t_limit) break;
+    if (vi[0] >= lmaxi[0] || vi[1] >= lmaxi[1] || vi[2] >= lmaxi[2]) break; /* t = t_limit */
+    const uint32_t diff = (prev[0] ^ vi[0]) | (prev[1] ^ vi[1]) | (prev[2] ^ vi[2]);
+    memcpy(prev, vi, sizeof prev);
+    arc_node64 node;
+    uint32_t shift;
+    if (diff < 4u && cached_shift == 0u) { /* :1089-1102 */
+      node = l3;
+      shift = 0u;
+    } else if (diff < 16u && cached_shift <= 2u) {
+      node = l2;
+      shift = 2u;
+    } else if (diff < 64u && cached_shift <= 4u) {
+      node = l1;
+      shift = 4u;
+    } else {
+      node = w->nodes[root_idx];
+      shift = 6u;
+    }
+    cached_shift = shift;
+    const uint32_t l[3] = {vi[0] & 255u, vi[1] & 255u, vi[2] & 255u};
+    for (;;) { /* :1106-1190 */
+      const uint32_t cx = (l[0] >> shift) & 3u, cy = (l[1] >> shift) & 3u, cz = (l[2] >> shift) & 3u;
+      const uint32_t b = (cz << 4) | (cy << 2) | cx;
+      const uint32_t span = 1u << shift;
+      uint32_t cmin[3], cmax[3];
+      for (int a = 0; a < 3; ++a) {
+        cmin[a] = vi[a] & ~(span - 1u);
+        cmax[a] = cmin[a] + span;
+      }
+      int step_box = 0;
+      if (!has_child(&node, b)) {
+        step_box = 1;
+      } else if ((node.child_ptr & ARC_IS_LEAF_FLAG) != 0u) {
+        uint32_t mat = (node.lod_payload >> 24) & 0xFFu;
+        if ((node.child_ptr & ARC_IS_UNIFORM_FLAG) == 0u) {
+          const uint32_t bi = node.child_ptr & ARC_PTR_INDEX_MASK;
+          mat = bi < w->n_bricks ? brick_mat(w, bi, b) : ARC_MAT_AIR;
+        }
+        if (mat != ARC_MAT_AIR) { /* :1143-1182 */
+          float tm[3];
+          for (int a = 0; a < 3; ++a) {
+            const uint32_t ec = pos[a] ? cmin[a] : cmax[a];
+            const float m = (float)ec * inv[a];
+            tm[a] = m + noi[a];
+          }
+          const float cell_enter = fmaxf(fmaxf(tm[0], tm[1]), tm[2]);
+          const float exact_t = fmaxf(cell_enter, 0.0f);
+          float hv[4], hm[4];
+          for (int a = 0; a < 3; ++a) {
+            const float q = ld[a] * exact_t;
+            hv[a] = vo[a] + q;
+            hm[a] = hv[a] * 0.0625f;
+          }
+          hm[3] = 1.0f;
+          res->hit = 1u;
+          res->t_voxels = exact_t;
+          mat_vec(rb->local_to_world, hm, res->hit_pos_m);
+          mat_vec(rb->prev_local_to_world, hm, res->prev_hit_pos_m);
+          float ln[4] = {0.0f, 1.0f, 0.0f, 0.0f};
+          if (tm[0] >= tm[1] && tm[0] >= tm[2]) {
+            ln[0] = -sign_wgsl(ld[0]);
+            ln[1] = 0.0f;
+          } else if (tm[1] >= tm[2]) {
+            ln[1] = -sign_wgsl(ld[1]);
+          } else {
+            ln[1] = 0.0f;
+            ln[2] = -sign_wgsl(ld[2]);
+          }
+          float wn[3];
+          mat_vec(rb->local_to_world, ln, wn);
+          const float len = sqrtf(dot3(wn, wn));
+          for (int a = 0; a < 3; ++a) res->normal[a] = wn[a] / len;
+          res->mat = mat;
+          res->voxel_hash = pcg_hash((vi[0] * 73856093u) ^ (vi[1] * 19349663u) ^ (vi[2] * 83492791u));
+          res->contact_ao = 1.0f;
+          if (shift == 0u) {
+            const float nv[3] = {ln[0], ln[1], ln[2]};
+            res->contact_ao = eval_leaf_ao(node.mask_low, node.mask_high, cx, cy, cz, nv, hv);
+          }
+          res->body = rb_idx;
+          return;
+        }
+        step_box = 1;
+      }
+      if (step_box) {
+        step_res st = step_dda_box(vo, ld, inv, noi, pos, cmin, cmax, t);
+        t = st.t;
+        memcpy(vi, st.vi, sizeof vi);
+        break;
+      }
+      const uint32_t next = child_offset(&node, b);
+      if (next >= w->n_nodes) return; /* tree leaves the pool: traced as empty */
+      node = w->nodes[next];
+      shift -= 2u;
+      cached_shift = shift;
+      if (shift == 4u) l1 = node;
+      else if (shift == 2u) l2 = node;
+      else if (shift == 0u) l3 = node;
+    }
+  }
+}
+
+static void rigid_all(const orc_world* w, const arc_rigid_body* rbs, uint32_t n_rb, const arc_ray* ray,
+                      arc_rigid_hit* best) {
+  const float o[3] = {ray->ox, ray->oy, ray->oz}, d[3] = {ray->dx, ray->dy, ray->dz};
+  rigid_miss(o, d, ray->t_max, best);
+  float cur_max_m = ray->t_max;
+  const uint32_t count = n_rb < ARC_MAX_RIGID_BODIES ? n_rb : ARC_MAX_RIGID_BODIES;
+  for (uint32_t i = 0; i < count; ++i) { /* :1232-1241 */
+    arc_rigid_hit cand;
+    rigid_single(w, &rbs[i], i, o, d, cur_max_m, &cand);
+    if (cand.hit && cand.t_voxels < best->t_voxels) {
+      *best = cand;
+      cur_max_m = cand.t_voxels * 0.0625f;
+    }
+  }
+}
+
+void orc_trace_rigid(const orc_world* w, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* rays,
+                     uint64_t n, arc_rigid_hit* out) {
+  for (uint64_t i = 0; i < n; ++i) rigid_all(w, bodies, n_bodies, &rays[i], &out[i]);
+}
+
+/* query_voxel_world_meters (traversal64.wgsl:182-220) */
+uint32_t orc_query_point(const orc_world* w, const float p[3]) {
+  float vp[3];
+  for (int a = 0; a < 3; ++a) {
+    vp[a] = p[a] * 16.0f;
+    const float wm = (float)w->grid[a] * 256.0f;
+    if (vp[a] < 0.0f || vp[a] >= wm) return ARC_MAT_AIR;
+  }
+  return orc_query_voxel(w, f2u(vp[0]), f2u(vp[1]), f2u(vp[2]));
+}
+
 /* ---- raycast_chunk_tree: raycast_cpu.rs:28-218 ------------------------------------------ */
 /* Operates on chunk 0 of the pools (root = roots[0]); pool indices are global, which is the
  * relocated image of the chunk's local nodes (world_map.rs:705-735). */

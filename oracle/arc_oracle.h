@@ -11,6 +11,8 @@
  *   orc_trace_meters  WorldMap::raycast_meters  crates/arcadia-world/src/world_map.rs:464-545
  *   orc_query_voxel   query_voxel_world_meters  shaders/common/traversal64.wgsl:182-220
  *   orc_trace_shadow  trace_shadow_vis / _di    shaders/common/traversal64.wgsl:571-773 / 781-989
+ *   orc_trace_rigid   trace_rigid_bodies        shaders/common/traversal64.wgsl:1005-1244
+ *   orc_query_point   query_voxel_world_meters  shaders/common/traversal64.wgsl:182-220
  * plus an independent checker that shares no code with the above:
  *   orc_brute_dense   Amanatides-Woo unit-voxel walk in double precision over a dense grid.
  *
@@ -64,6 +66,11 @@ void orc_trace_meters(const orc_world* w, const arc_ray* rays, uint64_t n, arc_t
  * record the kernel writes. */
 void orc_trace_shadow(const orc_world* w, const arc_shadow_params* sp, const arc_ray* rays, uint64_t n, float* vis,
                       arc_shadow_rec* rec, int nthreads);
+/* Rigid voxel bodies (first min(n_bodies, 32) of `bodies`), single-threaded; point query in
+ * world metres. */
+void orc_trace_rigid(const orc_world* w, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* rays,
+                     uint64_t n, arc_rigid_hit* out);
+uint32_t orc_query_point(const orc_world* w, const float p[3]);
 /* Compact records as the kernels write them. */
 void orc_compact(const arc_trace_result* full, uint64_t n, int world_units, arc_hit* out);
 

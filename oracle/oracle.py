@@ -76,6 +76,10 @@ def lib() -> C.CDLL:
         h.orc_sharc_query.argtypes = [C.c_uint32, P, P, C.c_uint64, P, P]
         for f in (h.orc_sharc_make_key, h.orc_sharc_deposit, h.orc_sharc_resolve, h.orc_sharc_query):
             f.restype = None
+        h.orc_trace_rigid.argtypes = [P, P, C.c_uint32, P, C.c_uint64, P]
+        h.orc_trace_rigid.restype = None
+        h.orc_query_point.argtypes = [P, P]
+        h.orc_query_point.restype = C.c_uint32
         h.orc_trace_shadow.argtypes = [P, P, P, C.c_uint64, P, P, C.c_int]
         h.orc_trace_shadow.restype = None
         h.orc_f32_to_f16.argtypes = [C.c_float]
@@ -151,6 +155,31 @@ def trace_shadow(world: World, mode: int, max_steps: int, rays: np.ndarray, nthr
         lib().orc_trace_shadow(world.ref, C.byref(sp), rays.ctypes.data, len(rays), vis.ctypes.data, rec.ctypes.data,
                                _threads(nthreads))
     return vis, rec
+
+
+RIGID_BODY_DTYPE = np.dtype([("world_to_local", "<f4", (16,)), ("local_to_world", "<f4", (16,)),
+                             ("prev_local_to_world", "<f4", (16,)), ("local_bounds_and_root", "<f4", (4,)),
+                             ("world_sphere", "<f4", (4,))])
+RIGID_HIT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("prev_hit_pos_m", "<f4", (3,)),
+                            ("normal", "<f4", (3,)), ("contact_ao", "<f4"), ("mat", "<u4"), ("voxel_hash", "<u4"),
+                            ("hit", "<u4"), ("body", "<u4"), ("reserved", "<u4")])
+
+
+def trace_rigid(world: World, bodies: np.ndarray, rays: np.ndarray) -> np.ndarray:
+    """trace_rigid_bodies over `rays` with the first min(len(bodies), 32) instances."""
+    rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+    b = np.ascontiguousarray(bodies, RIGID_BODY_DTYPE)
+    out = np.zeros(len(rays), RIGID_HIT_DTYPE)
+    if len(rays):
+        lib().orc_trace_rigid(world.ref, b.ctypes.data if len(b) else None, len(b), rays.ctypes.data, len(rays),
+                              out.ctypes.data)
+    return out
+
+
+def query_points(world: World, points: np.ndarray) -> np.ndarray:
+    """query_voxel_world_meters at each (x, y, z) metres."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    return np.array([lib().orc_query_point(world.ref, p.ctypes.data) for p in pts], np.uint32)
 
 
 def trace_meters(world: World, rays: np.ndarray, nthreads: int = 0):
