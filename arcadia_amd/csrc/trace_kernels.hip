@@ -60,25 +60,27 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh (empty chunk: 256)
   bool cand = false;  // set bit at a leaf or at the LOD floor: hit unless the voxel is air/water
   if (descend) {
-    for (;;) {  // descent (:390-552), at most four levels
+    // descent (:390-552), at most four levels, written with a single loop exit (one exec-mask
+    // update per level instead of one per way out)
+    bool go;
+    do {
       const uint32_t b = child_bit(L, sh);
-      if (SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
-          (node.w >> 24) == ARC_MAT_WATER) {
-        span_mul = 4u;
-        break;
-      }
+      const bool water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
+                         (node.w >> 24) == ARC_MAT_WATER;
       const uint64_t below = (((uint64_t)node.y << 32) | node.x) << (63u - b);
-      if ((int64_t)below >= 0) break;  // no child at b
+      const bool has = (int64_t)below < 0 && !water;  // child at b
       // Level-3 nodes are leaves in every pool the upload check accepts; sh == 0 (implied by
       // sh <= minsh) only bounds the descent if a malformed node slipped in through a patch.
-      if ((node.z & ARC_IS_LEAF_FLAG) != 0u || sh <= minsh) {
-        cand = true;
-        break;
+      const bool stop = (node.z & ARC_IS_LEAF_FLAG) != 0u || sh <= minsh;
+      if (water) span_mul = 4u;
+      cand = has && stop;
+      go = has && !stop;
+      if (go) {
+        node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
+        sh -= 2u;
+        if (sh != 0u) *cold_slot(C, sh) = node;
       }
-      node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
-      sh -= 2u;
-      if (sh != 0u) *cold_slot(C, sh) = node;
-    }
+    } while (go);
   }
   if (cand) {  // :441-538
     const uint32_t b = child_bit(L, sh);
