@@ -20,7 +20,7 @@ OBJ = PKG / "_build"
 LIB = PKG / "libarcadia_hip.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
-HOST_SOURCES = ["world_build.cpp", "world_edit.cpp", "patch_plan.cpp", "worldgen.cpp", "rays.cpp"]
+HOST_SOURCES = ["world_build.cpp", "world_edit.cpp", "patch_plan.cpp", "worldgen.cpp", "rays.cpp", "snapshot.cpp"]
 HIP_HOST_SOURCES = ["api.cpp"]
 HIP_SOURCES = ["trace_kernels.hip", "shadow.hip", "rigid.hip", "gbuffer.hip", "patch.hip", "sharc.hip", "pathtrace.hip", "nrc.hip"]
 
@@ -56,7 +56,7 @@ def build(verbose: bool = False) -> Path:
         objs.append(str(o))
     for src in HIP_HOST_SOURCES:
         o = OBJ / (src + ".o")
-        _run(["g++", *COMMON, "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", "-c", str(CSRC / src), "-o", str(o)])
+        _run(["g++", *COMMON, "-pthread", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}", "-c", str(CSRC / src), "-o", str(o)])
         objs.append(str(o))
     for src in HIP_SOURCES:
         o = OBJ / (src + ".o")

@@ -3,11 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/arcadia_hip.h"
@@ -16,6 +18,7 @@
 #include "patch.h"
 #include "patch_plan.h"
 #include "pathtrace.h"
+#include "snapshot.h"
 #include "trace_launch.h"
 
 struct arc_ctx {
@@ -51,6 +54,11 @@ struct arc_ctx {
   void* h_stage = nullptr;
   void* d_stage = nullptr;
   size_t stage_bytes = 0;
+  // arc_snapshot_upload staging: two pinned batch buffers and their device chunk tables, kept
+  // for later uploads (page-locking is a large part of a cold upload)
+  uint8_t* h_snap[2] = {nullptr, nullptr};
+  uint4* d_snap_table[2] = {nullptr, nullptr};
+  size_t snap_bytes = 0, snap_table_bytes = 0;
 };
 
 namespace {
@@ -89,47 +97,6 @@ arc_launch_config default_config() {
 }
 
 uint32_t lod_class(const arc_trace_params* p) { return p->lod_bias == 0u ? 0u : (p->lod_bias == 1u ? 1u : 2u); }
-
-// Structural check of uploaded pools: every reachable node's children lie inside the node
-// pool, mixed leaves point inside the brick pool, and trees are at most four levels deep with
-// leaves at the last level. Malformed pools are refused before any kernel reads them.
-int check_tree(const arc_node64* nodes, uint64_t n_nodes, uint64_t n_bricks, uint32_t root, std::string& why) {
-  struct Item {
-    uint32_t idx, level;
-  };
-  std::vector<Item> stack;
-  stack.push_back({root, 0});
-  while (!stack.empty()) {
-    Item it = stack.back();
-    stack.pop_back();
-    if (it.idx >= n_nodes) {
-      why = "node index " + std::to_string(it.idx) + " outside the node pool";
-      return ARC_ERR_INVALID;
-    }
-    const arc_node64& n = nodes[it.idx];
-    const uint64_t mask = (uint64_t)n.mask_low | ((uint64_t)n.mask_high << 32);
-    if (mask == 0) continue;
-    const uint32_t base = n.child_ptr & ARC_PTR_INDEX_MASK;
-    if (n.child_ptr & ARC_IS_LEAF_FLAG) {
-      if (!(n.child_ptr & ARC_IS_UNIFORM_FLAG) && base >= n_bricks) {
-        why = "brick index " + std::to_string(base) + " outside the brick pool";
-        return ARC_ERR_INVALID;
-      }
-      continue;
-    }
-    if (it.level >= 3) {
-      why = "internal node " + std::to_string(it.idx) + " below level 3";
-      return ARC_ERR_INVALID;
-    }
-    const uint32_t cnt = (uint32_t)__builtin_popcountll(mask);
-    if ((uint64_t)base + cnt > n_nodes) {
-      why = "children of node " + std::to_string(it.idx) + " outside the node pool";
-      return ARC_ERR_INVALID;
-    }
-    for (uint32_t k = 0; k < cnt; ++k) stack.push_back({base + k, it.level + 1});
-  }
-  return ARC_OK;
-}
 
 bool valid_params(const arc_trace_params* p) {
   if (!p) return false;
@@ -199,6 +166,10 @@ void arc_destroy(arc_ctx* c) {
   (void)hipFree(c->d_gb_res);
   (void)hipFree(c->d_stage);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  for (int i = 0; i < 2; ++i) {
+    if (c->h_snap[i]) (void)hipHostFree(c->h_snap[i]);
+    (void)hipFree(c->d_snap_table[i]);
+  }
   delete c;
 }
 
@@ -266,7 +237,7 @@ int arc_world_upload(arc_ctx* c, const uint32_t grid[3], const uint32_t* roots, 
   for (uint64_t i = 0; i < nchunks; ++i) {
     if (roots[i] == ARC_EMPTY_CHUNK_SENTINEL) continue;
     std::string why;
-    if (check_tree(nodes, n_nodes, n_bricks, roots[i], why) != ARC_OK)
+    if (arc::check_tree(nodes, n_nodes, n_bricks, roots[i], why) != ARC_OK)
       return fail(c, ARC_ERR_INVALID, "chunk %llu: %s", (unsigned long long)i, why.c_str());
   }
   HIP_TRY(c, hipSetDevice(c->device));
@@ -289,6 +260,162 @@ int arc_world_upload(arc_ctx* c, const uint32_t grid[3], const uint32_t* roots, 
     HIP_TRY(c, hipMemset(c->d_bricks + n_bricks, 0, (brick_cap - n_bricks) * sizeof(arc_brick64)));
   HIP_TRY(c, hipDeviceSynchronize());
   std::memcpy(c->grid, grid, sizeof c->grid);
+  c->n_nodes = n_nodes;
+  c->node_cap = node_cap;
+  c->n_bricks = n_bricks;
+  c->brick_cap = brick_cap;
+  c->has_world = true;
+  c->world_gen++;
+  return ARC_OK;
+}
+
+// Streaming pool builder: a snapshot's chunks, in their stored (Morton) order, go straight to
+// their final pool offsets. Batches of up to 128 MiB are staged in two pinned buffers: while the
+// DMA engine copies one batch, host threads check (CRC, tree rules) and stage the next. A small
+// kernel then rebases the batch's node pointers in HBM. The pools equal those of
+// arc_world_host_pack followed by arc_world_upload, without building them on the host.
+int arc_snapshot_upload(arc_ctx* c, const arc_snapshot* sh, uint64_t node_cap, uint64_t brick_cap, int nthreads) {
+  if (!c || !sh) return fail(c, ARC_ERR_INVALID, "arc_snapshot_upload: null argument");
+  const arc::Snapshot& s = *reinterpret_cast<const arc::Snapshot*>(sh);
+  const arc::SnapHeader& h = s.h;
+  const uint64_t n_nodes = h.total_nodes + 1, n_bricks = h.total_bricks + 1;  // slot-0 dummies
+  if (node_cap < n_nodes) node_cap = n_nodes;
+  if (brick_cap < n_bricks) brick_cap = n_bricks;
+  if (node_cap > (uint64_t)ARC_PTR_INDEX_MASK + 1 || brick_cap > (uint64_t)ARC_PTR_INDEX_MASK + 1)
+    return fail(c, ARC_ERR_INVALID, "pool capacity beyond the 30-bit pointer range");
+  const uint64_t nchunks = (uint64_t)h.grid[0] * h.grid[1] * h.grid[2];
+  // batches of consecutive chunks
+  struct Batch {
+    uint32_t k0, k1;
+    uint64_t node0, brick0, n_nodes, n_bricks;
+  };
+  constexpr uint64_t kBatchBytes = 128ull << 20;
+  std::vector<Batch> batches;
+  std::vector<uint32_t> roots(nchunks, ARC_EMPTY_CHUNK_SENTINEL);
+  std::vector<uint64_t> node_at(h.n_present), brick_at(h.n_present);
+  uint64_t stage_max = 0, nb = 1, bb = 1;
+  uint32_t table_max = 0;
+  for (uint32_t k = 0; k < h.n_present; ++k) {
+    const arc::SnapChunk& e = s.index[k];
+    const uint64_t bytes = arc::snap_payload_bytes(e);
+    if (batches.empty() || (batches.back().n_nodes * 16 + batches.back().n_bricks * 64 + bytes > kBatchBytes &&
+                            batches.back().k1 > batches.back().k0))
+      batches.push_back(Batch{k, k, nb, bb, 0, 0});
+    Batch& b = batches.back();
+    b.k1 = k + 1;
+    b.n_nodes += e.n_nodes;
+    b.n_bricks += e.n_bricks;
+    stage_max = std::max(stage_max, b.n_nodes * 16 + b.n_bricks * 64);
+    table_max = std::max(table_max, b.k1 - b.k0);
+    node_at[k] = nb;
+    brick_at[k] = bb;
+    roots[e.chunk] = (uint32_t)nb;
+    nb += e.n_nodes;
+    bb += e.n_bricks;
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipDeviceSynchronize());
+  (void)hipFree(c->d_roots);
+  (void)hipFree(c->d_nodes);
+  (void)hipFree(c->d_bricks);
+  c->d_roots = nullptr;
+  c->d_nodes = nullptr;
+  c->d_bricks = nullptr;
+  c->has_world = false;
+  HIP_TRY(c, hipMalloc(&c->d_roots, nchunks * sizeof(uint32_t)));
+  HIP_TRY(c, hipMalloc(&c->d_nodes, node_cap * sizeof(arc_node64)));
+  HIP_TRY(c, hipMalloc(&c->d_bricks, brick_cap * sizeof(arc_brick64)));
+  // staging: the context's pinned buffers (grown when too small) and two events, destroyed on
+  // every exit after the copies that read the buffers have finished
+  struct Events {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Events() {
+      for (auto& e : ev)
+        if (e) {
+          (void)hipEventSynchronize(e);
+          (void)hipEventDestroy(e);
+        }
+    }
+  } st;
+  const size_t table_bytes = ((size_t)std::max<uint32_t>(table_max, 1u) * sizeof(uint4) + 255) & ~(size_t)255;
+  if (!batches.empty()) {
+    if (c->snap_bytes < table_bytes + stage_max || c->snap_table_bytes < table_bytes) {
+      for (int i = 0; i < 2; ++i) {
+        if (c->h_snap[i]) (void)hipHostFree(c->h_snap[i]);
+        (void)hipFree(c->d_snap_table[i]);
+        c->h_snap[i] = nullptr;
+        c->d_snap_table[i] = nullptr;
+      }
+      c->snap_bytes = c->snap_table_bytes = 0;
+      for (int i = 0; i < 2; ++i) {
+        HIP_TRY(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_snap[i]), table_bytes + stage_max, hipHostMallocDefault));
+        HIP_TRY(c, hipMalloc(&c->d_snap_table[i], table_bytes));
+      }
+      c->snap_bytes = table_bytes + stage_max;
+      c->snap_table_bytes = table_bytes;
+    }
+    for (int i = 0; i < 2; ++i) HIP_TRY(c, hipEventCreateWithFlags(&st.ev[i], hipEventDisableTiming));
+  }
+  const hipStream_t stream = nullptr;
+  const int nt = std::max(1, std::min(nthreads, 64));
+  for (size_t bi = 0; bi < batches.size(); ++bi) {
+    const Batch& b = batches[bi];
+    const int slot = (int)(bi & 1);
+    HIP_TRY(c, hipEventSynchronize(st.ev[slot]));  // the DMA of batch bi - 2 has left this buffer
+    uint8_t* const tab = c->h_snap[slot];
+    uint8_t* const nodes_h = tab + table_bytes;
+    uint8_t* const bricks_h = nodes_h + b.n_nodes * 16;
+    std::vector<std::string> why(b.k1 - b.k0);
+    std::atomic<uint32_t> next{b.k0};
+    std::atomic<int> status{ARC_OK};
+    auto work = [&]() {
+      for (uint32_t k; (k = next.fetch_add(1)) < b.k1 && status.load() == ARC_OK;) {
+        if (arc::snap_check_chunk(s, k, why[k - b.k0]) != ARC_OK) {
+          status.store(ARC_ERR_CORRUPT);
+          return;
+        }
+        const arc::SnapChunk& e = s.index[k];
+        const uint8_t* p = arc::snap_payload(s, k);
+        std::memcpy(nodes_h + (node_at[k] - b.node0) * 16, p, (size_t)e.n_nodes * 16);
+        std::memcpy(bricks_h + (brick_at[k] - b.brick0) * 64, p + (size_t)e.n_nodes * 16, (size_t)e.n_bricks * 64);
+        const uint4 t = make_uint4((uint32_t)node_at[k], e.n_nodes, (uint32_t)brick_at[k], 0u);
+        std::memcpy(tab + (size_t)(k - b.k0) * sizeof(uint4), &t, sizeof t);
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < std::min<int>(nt, (int)(b.k1 - b.k0)); ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    if (status.load() != ARC_OK) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipFree(c->d_roots);
+      (void)hipFree(c->d_nodes);
+      (void)hipFree(c->d_bricks);
+      c->d_roots = nullptr;
+      c->d_nodes = nullptr;
+      c->d_bricks = nullptr;
+      for (auto& w : why)
+        if (!w.empty()) return fail(c, ARC_ERR_CORRUPT, "%s: %s", s.path.c_str(), w.c_str());
+      return fail(c, ARC_ERR_CORRUPT, "%s: corrupt chunk", s.path.c_str());
+    }
+    const uint32_t nt_b = b.k1 - b.k0;
+    HIP_TRY(c, hipMemcpyAsync(c->d_snap_table[slot], tab, nt_b * sizeof(uint4), hipMemcpyHostToDevice, stream));
+    HIP_TRY(c, hipMemcpyAsync(c->d_nodes + b.node0, nodes_h, b.n_nodes * 16, hipMemcpyHostToDevice, stream));
+    if (b.n_bricks)
+      HIP_TRY(c, hipMemcpyAsync(c->d_bricks + b.brick0, bricks_h, b.n_bricks * 64, hipMemcpyHostToDevice, stream));
+    HIP_TRY(c, arcdev::launch_relocate_chunks(c->d_nodes, (uint32_t)b.node0, (uint32_t)(b.node0 + b.n_nodes),
+                                              c->d_snap_table[slot], nt_b, stream));
+    HIP_TRY(c, hipEventRecord(st.ev[slot], stream));
+  }
+  HIP_TRY(c, hipMemsetAsync(c->d_nodes, 0, sizeof(arc_node64), stream));
+  HIP_TRY(c, hipMemsetAsync(c->d_bricks, 0, sizeof(arc_brick64), stream));
+  if (node_cap > n_nodes)
+    HIP_TRY(c, hipMemsetAsync(c->d_nodes + n_nodes, 0, (node_cap - n_nodes) * sizeof(arc_node64), stream));
+  if (brick_cap > n_bricks)
+    HIP_TRY(c, hipMemsetAsync(c->d_bricks + n_bricks, 0, (brick_cap - n_bricks) * sizeof(arc_brick64), stream));
+  HIP_TRY(c, hipMemcpy(c->d_roots, roots.data(), nchunks * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipDeviceSynchronize());
+  std::memcpy(c->grid, h.grid, sizeof c->grid);
   c->n_nodes = n_nodes;
   c->node_cap = node_cap;
   c->n_bricks = n_bricks;

@@ -18,6 +18,8 @@ ARC_ERR_INVALID = -1
 ARC_ERR_HIP = -2
 ARC_ERR_NO_WORLD = -3
 ARC_ERR_CAPACITY = -4
+ARC_ERR_IO = -5
+ARC_ERR_CORRUPT = -6
 
 ARC_SEM_WORLD = 0
 ARC_SEM_CHUNK = 1
@@ -315,6 +317,16 @@ _SIGS = [
     ("arc_gen_view_rays", C.c_int64, [C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_float, C.c_void_p]),
     ("arc_gen_bounce_rays_dense", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_float,
                                             C.c_void_p]),
+    ("arc_snapshot_write", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("arc_snapshot_open", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("arc_snapshot_close", None, [C.c_void_p]),
+    ("arc_snapshot_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("arc_snapshot_verify", C.c_int, [C.c_void_p, C.c_int]),
+    ("arc_snapshot_load_host", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    ("arc_snapshot_last_error", C.c_char_p, []),
+    ("arc_snapshot_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_int]),
+    ("arc_crc32", C.c_uint32, [C.c_void_p, C.c_uint64, C.c_uint32]),
     ("arc_gen_bounce_rays", C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_float, C.c_void_p,
                                         C.c_void_p]),
 ]
@@ -440,6 +452,19 @@ class HostWorld:
         rc = lib().arc_world_host_pack(self._h)
         if rc != ARC_OK:
             raise ArcError(rc, "pack")
+
+    @classmethod
+    def _adopt(cls, handle, grid) -> "HostWorld":
+        w = cls.__new__(cls)
+        w.grid = tuple(int(g) for g in grid)
+        w._h = handle
+        return w
+
+    def save(self, path) -> None:
+        """Write the chunk trees as a world snapshot (arc_snapshot_write)."""
+        rc = lib().arc_snapshot_write(self._h, str(path).encode())
+        if rc != ARC_OK:
+            raise ArcError(rc, f"arc_snapshot_write: {lib().arc_snapshot_last_error().decode()}")
 
     # ---- in-place edits (world_edit.cpp) ----
     def _rc(self, rc, what):
@@ -629,6 +654,55 @@ def bounce_rays(prays: np.ndarray, hits: np.ndarray, seed: int = 1, max_dist_m: 
 # ---------------------------------------------------------------------------------------------
 # device context
 # ---------------------------------------------------------------------------------------------
+class Snapshot:
+    """An open world snapshot file (arc_snapshot_open): header, index and checksums checked."""
+
+    def __init__(self, path):
+        self.path = str(path)
+        h = C.c_void_p()
+        rc = lib().arc_snapshot_open(self.path.encode(), C.byref(h))
+        if rc != ARC_OK:
+            raise ArcError(rc, f"arc_snapshot_open: {lib().arc_snapshot_last_error().decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().arc_snapshot_close(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        g = (C.c_uint32 * 3)()
+        n, tn, tb = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        lib().arc_snapshot_info(self._h, g, C.byref(n), C.byref(tn), C.byref(tb))
+        return {"grid": tuple(g), "chunks": n.value, "nodes": tn.value, "bricks": tb.value}
+
+    def verify(self, nthreads: int = 8) -> None:
+        rc = lib().arc_snapshot_verify(self._h, nthreads)
+        if rc != ARC_OK:
+            raise ArcError(rc, f"arc_snapshot_verify: {lib().arc_snapshot_last_error().decode()}")
+
+    def load_host(self, nthreads: int = 8) -> HostWorld:
+        h = C.c_void_p()
+        rc = lib().arc_snapshot_load_host(self._h, nthreads, C.byref(h))
+        if rc != ARC_OK:
+            raise ArcError(rc, f"arc_snapshot_load_host: {lib().arc_snapshot_last_error().decode()}")
+        return HostWorld._adopt(h, self.info()["grid"])
+
+
+def crc32(data, crc: int = 0) -> int:
+    """The snapshot checksum (zlib.crc32) computed by the library."""
+    b = np.frombuffer(bytes(data), np.uint8) if not isinstance(data, np.ndarray) else np.ascontiguousarray(data)
+    return int(lib().arc_crc32(b.ctypes.data if b.nbytes else None, b.nbytes, crc))
+
+
 class Context:
     """A device context holding the world pools (Renderer's Group0Resources equivalent)."""
 
@@ -665,6 +739,12 @@ class Context:
 
     def upload_world(self, world: HostWorld, **kw):
         self.upload(*world.pools(), **kw)
+
+    def upload_snapshot(self, snap: Snapshot, node_capacity: int = 0, brick_capacity: int = 0,
+                        nthreads: int = 8) -> None:
+        """Stream a snapshot's chunks into the device pools (arc_snapshot_upload)."""
+        self._check(lib().arc_snapshot_upload(self._h, snap._h, node_capacity, brick_capacity, nthreads),
+                    "arc_snapshot_upload")
 
     def batch(self, views, primary: TraceParams, bounce: TraceParams, n_bounces: int = 2,
               bounce_max_dist_m: float = 26.0, seed: int = 1) -> "Batch":

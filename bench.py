@@ -307,6 +307,57 @@ def rigid_leg(args, ctx, pools, prays, phits, stream, threads):
     return res
 
 
+def snapshot_leg(args, ctx, world, pools, threads):
+    """World snapshot of the full-frame world: written once (untimed), then loaded three ways,
+    each timed to resident device pools: the streaming pool builder (arc_snapshot_upload, file
+    pages already cached), the same after a host load + pack, and arc_world_upload of the
+    already-packed host pools. The streamed pools are read back and compared with the host pools."""
+    import tempfile
+
+    import arcadia_amd as A
+
+    grid, roots, nodes, bricks = pools
+    pool_bytes = len(nodes) * 16 + len(bricks) * 64
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "world.arcw")
+        t = time.perf_counter()
+        world.save(path)
+        t_write = time.perf_counter() - t
+        file_bytes = os.path.getsize(path)
+        res = {"metric": "world snapshot load to resident device pools (full-frame world)",
+               "pool_bytes": pool_bytes, "file_bytes": file_bytes, "write_s": round(t_write, 3)}
+        with A.Snapshot(path) as s:
+            s.verify(threads)  # also pulls the file into the page cache
+            times = []
+            for _ in range(3):
+                t = time.perf_counter()
+                ctx.upload_snapshot(s, nthreads=threads)
+                times.append(time.perf_counter() - t)
+            res["stream_s"] = round(float(np.median(times)), 4)
+            r2, n2, b2 = ctx.read_pools(len(roots), len(nodes), len(bricks))
+            res["stream_pools_equal"] = bool(np.array_equal(r2, roots) and
+                                             np.array_equal(n2.view(np.uint32), nodes.view(np.uint32)) and
+                                             np.array_equal(b2.view(np.uint32), bricks.view(np.uint32)))
+            t = time.perf_counter()
+            w2 = s.load_host(threads)
+            w2.pack()
+            ctx.upload_world(w2)
+            res["host_load_pack_upload_s"] = round(time.perf_counter() - t, 4)
+            w2.close()
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        ctx.upload(grid, roots, nodes, bricks)
+        times.append(time.perf_counter() - t)
+    res["upload_packed_pools_s"] = round(float(np.median(times)), 4)
+    res["stream_gbs"] = round(pool_bytes / res["stream_s"] / 1e9, 2)
+    print(f"[bench] snapshot: {pool_bytes / 1e6:.0f} MB pools; stream {res['stream_s']} s "
+          f"({res['stream_gbs']} GB/s, equal {res['stream_pools_equal']}), host load+pack+upload "
+          f"{res['host_load_pack_upload_s']} s, upload of packed pools {res['upload_packed_pools_s']} s",
+          file=sys.stderr, flush=True)
+    return res
+
+
 def cache_leg(args, threads):
     """Learned radiance cache vs SHARC (2^22 entries) vs no cache, rel-MSE against the FP64
     1024-spp path-traced judge at 1920x1080 on the full-frame world with 8 emissive clusters."""
@@ -484,6 +535,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-walker workload")
     ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-body workload")
+    ap.add_argument("--no-snapshot", action="store_true", help="skip the world-snapshot workload")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache workload")
     ap.add_argument("--cache-ref-spp", type=int, default=1024)
     ap.add_argument("--cache-frames", type=int, default=60)
@@ -501,6 +553,7 @@ def main() -> int:
         args.no_batched = True
         args.no_shadow = True
         args.no_rigid = True
+        args.no_snapshot = True
         args.no_cache = True
         args.no_destruction = True
 
@@ -633,6 +686,8 @@ def main() -> int:
         out["shadow"] = shadow_leg(args, ctx, pools, prays, phits, stream, threads)
     if not args.no_rigid:
         out["rigid"] = rigid_leg(args, ctx, pools, prays, phits, stream, threads)
+    if not args.no_snapshot:
+        out["snapshot"] = snapshot_leg(args, ctx, world, pools, threads)
     ctx.close()
     del d_prays, d_phits, d_brays, d_bhits
     if not args.no_batched:

@@ -26,6 +26,8 @@ extern "C" {
 #define ARC_ERR_HIP (-2)       /* a HIP runtime call failed */
 #define ARC_ERR_NO_WORLD (-3)  /* trace before arc_world_upload */
 #define ARC_ERR_CAPACITY (-4)  /* patch or batch exceeds the device pool capacity */
+#define ARC_ERR_IO (-5)        /* a snapshot file could not be read or written */
+#define ARC_ERR_CORRUPT (-6)   /* a snapshot file fails its format, checksum or tree checks */
 
 /* ---- layout constants (crates/arcadia-core/src/node64.rs:9-16, world_map.rs:16) ----- */
 #define ARC_IS_LEAF_FLAG 0x80000000u
@@ -176,6 +178,18 @@ int arc_world_patch(arc_ctx* ctx, const arc_gpu_patch* patches, uint64_t n);
  * a patch writes past the given capacities. */
 int arc_world_patch_host(const arc_gpu_patch* patches, uint64_t n, uint32_t n_chunks, uint32_t* roots,
                          arc_node64* nodes, uint64_t node_cap, arc_brick64* bricks, uint64_t brick_cap);
+/* World snapshots (no reference counterpart: the reference keeps worlds in memory only,
+ * ARCHITECTURE.md:243). An open snapshot file; see arcadia_world.h for writing and opening. */
+typedef struct arc_snapshot arc_snapshot;
+/* Streaming pool builder: uploads a snapshot's chunks straight to their final pool offsets (the
+ * layout of WorldMap::rebuild_all_gpu_pools, world_map.rs:158-205). Batches of up to 128 MiB are
+ * staged through two pinned buffers (kept by the context for later uploads), filled by `nthreads`
+ * host threads while the previous batch is copied; the node pointers are then rebased on the
+ * device. Every chunk's checksum and tree rules are checked as it is staged (ARC_ERR_CORRUPT: the
+ * context is then left without a world). The resulting pools equal arc_world_host_pack +
+ * arc_world_upload of the same world. Blocking. */
+int arc_snapshot_upload(arc_ctx* ctx, const arc_snapshot* snap, uint64_t node_capacity, uint64_t brick_capacity,
+                        int nthreads);
 /* Device addresses of the pools (for callers that build their own launches). */
 int arc_world_device_pools(const arc_ctx* ctx, const uint32_t** roots, const arc_node64** nodes,
                            const arc_brick64** bricks);

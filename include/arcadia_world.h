@@ -163,6 +163,30 @@ int arc_world_host_carve_crater(arc_world_host* w, const float center_m[3], floa
 int arc_world_host_pending_patches(arc_world_host* w, const arc_gpu_patch** patches, uint64_t* n);
 int arc_world_host_clear_patches(arc_world_host* w);
 
+/* ---- world snapshots (new: the reference has no on-disk format, ARCHITECTURE.md:243) ---------
+ * A snapshot file holds every non-empty chunk's local 64-tree (nodes[0] = root, chunk-local
+ * pointers, then its bricks) in the Morton chunk order of rebuild_all_gpu_pools
+ * (world_map.rs:158-205), each payload page-aligned and covered by a CRC-32 (zlib's), behind a
+ * 64-byte header and a 32-byte-per-chunk index that carry their own CRC-32s.
+ *   arc_snapshot_write      writes the world's chunk trees (to path.tmp, then renamed over path).
+ *   arc_snapshot_open       maps the file and checks header, index order, offsets and checksums
+ *                           (ARC_ERR_IO: unreadable; ARC_ERR_CORRUPT: fails a check).
+ *   arc_snapshot_verify     checks every chunk payload (checksum and tree rules) on nthreads threads.
+ *   arc_snapshot_load_host  a host world with the stored chunk trees (checked as they are read);
+ *                           arc_world_host_pack then gives the original world's pools.
+ *   arc_snapshot_upload     (arcadia_hip.h) streams the chunks straight into the device pools.
+ * arc_snapshot_last_error gives the message of the calling thread's last failing snapshot call. */
+int arc_snapshot_write(const arc_world_host* w, const char* path);
+int arc_snapshot_open(const char* path, arc_snapshot** out);
+void arc_snapshot_close(arc_snapshot* snap);
+int arc_snapshot_info(const arc_snapshot* snap, uint32_t grid[3], uint32_t* n_chunks_stored, uint64_t* total_nodes,
+                      uint64_t* total_bricks);
+int arc_snapshot_verify(const arc_snapshot* snap, int nthreads);
+int arc_snapshot_load_host(const arc_snapshot* snap, int nthreads, arc_world_host** out);
+const char* arc_snapshot_last_error(void);
+/* CRC-32 as the snapshot format uses it (zlib.crc32(data, crc)). */
+uint32_t arc_crc32(const void* data, uint64_t n, uint32_t crc);
+
 #ifdef __cplusplus
 }
 #endif
