@@ -22,7 +22,7 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 HOST_SOURCES = ["world_build.cpp", "world_edit.cpp", "patch_plan.cpp", "worldgen.cpp", "rays.cpp", "snapshot.cpp"]
 HIP_HOST_SOURCES = ["api.cpp"]
-HIP_SOURCES = ["trace_kernels.hip", "shadow.hip", "rigid.hip", "gbuffer.hip", "patch.hip", "sharc.hip", "pathtrace.hip", "nrc.hip"]
+HIP_SOURCES = ["trace_kernels.hip", "shadow.hip", "rigid.hip", "gbuffer.hip", "patch.hip", "sharc.hip", "pathtrace.hip", "nrc.hip", "restir.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = [

@@ -18,6 +18,7 @@
 #include "patch.h"
 #include "patch_plan.h"
 #include "pathtrace.h"
+#include "restir.h"
 #include "snapshot.h"
 #include "trace_launch.h"
 
@@ -59,6 +60,9 @@ struct arc_ctx {
   uint8_t* h_snap[2] = {nullptr, nullptr};
   uint4* d_snap_table[2] = {nullptr, nullptr};
   size_t snap_bytes = 0, snap_table_bytes = 0;
+  // ReSTIR pass scratch: per-pixel ray and result buffers (grown to the largest frame seen)
+  uint8_t* d_restir = nullptr;
+  size_t restir_bytes = 0;
 };
 
 namespace {
@@ -170,6 +174,7 @@ void arc_destroy(arc_ctx* c) {
     if (c->h_snap[i]) (void)hipHostFree(c->h_snap[i]);
     (void)hipFree(c->d_snap_table[i]);
   }
+  (void)hipFree(c->d_restir);
   delete c;
 }
 
@@ -1065,6 +1070,159 @@ void arc_batch_destroy(arc_batch* b) {
   (void)hipFree(b->d_views);
   (void)hipFree(b->d_first);
   delete b;
+}
+
+
+// ---- ReSTIR GI / DI passes ---------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// Carves the context's ReSTIR scratch into the per-pixel buffers of `d` (256-byte aligned).
+int restir_scratch(arc_ctx* c, uint64_t n, arcdev::RestirDev& d) {
+  const size_t sizes[] = {sizeof(uint2),   sizeof(arc_ray),          sizeof(arc_ray),      sizeof(arc_ray),
+                          sizeof(arc_ray), sizeof(arc_hit),          sizeof(float),        sizeof(float),
+                          sizeof(arc_trace_result), sizeof(arc_rigid_hit), sizeof(arc_sharc_probe), sizeof(float4),
+                          sizeof(float)};
+  constexpr int kN = (int)(sizeof sizes / sizeof sizes[0]);
+  size_t off[kN], total = 0;
+  for (int k = 0; k < kN; ++k) {
+    off[k] = total;
+    total += (sizes[k] * n + 255) & ~(size_t)255;
+  }
+  if (total > c->restir_bytes) {
+    HIP_TRY(c, hipDeviceSynchronize());  // work queued on any stream may still use the old buffers
+    (void)hipFree(c->d_restir);
+    c->d_restir = nullptr;
+    c->restir_bytes = 0;
+    HIP_TRY(c, hipMalloc(&c->d_restir, total));
+    c->restir_bytes = total;
+  }
+  uint8_t* b = c->d_restir;
+  d.state = reinterpret_cast<uint2*>(b + off[0]);
+  d.rays_a = reinterpret_cast<arc_ray*>(b + off[1]);
+  d.rays_b = reinterpret_cast<arc_ray*>(b + off[2]);
+  d.rays_c = reinterpret_cast<arc_ray*>(b + off[3]);
+  d.rays_d = reinterpret_cast<arc_ray*>(b + off[4]);
+  d.hits_a = reinterpret_cast<const arc_hit*>(b + off[5]);
+  d.vis_a = reinterpret_cast<const float*>(b + off[6]);
+  d.vis_b = reinterpret_cast<const float*>(b + off[7]);
+  d.full_b = reinterpret_cast<const arc_trace_result*>(b + off[8]);
+  d.rb_c = reinterpret_cast<const arc_rigid_hit*>(b + off[9]);
+  d.probes = reinterpret_cast<arc_sharc_probe*>(b + off[10]);
+  d.sharc_out = reinterpret_cast<const float4*>(b + off[11]);
+  d.vis_d = reinterpret_cast<const float*>(b + off[12]);
+  return ARC_OK;
+}
+
+int restir_common(arc_ctx* c, const arc_restir_params* p, const uint32_t* dm, const uint32_t* nm,
+                  const arc_gi_reservoir* gi_in, arc_gi_reservoir* gi_out, const arc_di_reservoir* di_in,
+                  arc_di_reservoir* di_out, const arc_emissive_source* sources, arcdev::RestirDev& d) {
+  if (!c || !p) return ARC_ERR_INVALID;
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "ReSTIR pass before arc_world_upload");
+  if (p->width == 0 || p->height == 0 || p->width > 65535 || p->height > 65535)
+    return fail(c, ARC_ERR_INVALID, "width and height must be 1..65535 (world_grid_dims.w packs them in 16 bits)");
+  if (!dm || !nm || !gi_in || !gi_out || !di_in || !di_out)
+    return fail(c, ARC_ERR_INVALID, "null G-buffer or reservoir pointer");
+  if (p->n_emissive > 16384u) return fail(c, ARC_ERR_INVALID, "n_emissive above DI_MAX_SOURCES (16384)");
+  if (p->n_emissive && !sources) return fail(c, ARC_ERR_INVALID, "n_emissive > 0 needs the emissive sources");
+  if (p->shadow_mode > 1u) return fail(c, ARC_ERR_INVALID, "shadow_mode must be 0 or 1");
+  HIP_TRY(c, hipSetDevice(c->device));
+  const uint64_t n = (uint64_t)p->width * p->height;
+  int rc = restir_scratch(c, n, d);
+  if (rc != ARC_OK) return rc;
+  d.p = *p;
+  d.gx = c->grid[0];
+  d.gy = c->grid[1];
+  d.gz = c->grid[2];
+  d.roots = c->d_roots;
+  d.nodes = reinterpret_cast<const uint4*>(c->d_nodes);
+  d.palette = c->d_palette;
+  d.depth_mat = reinterpret_cast<const uint2*>(dm);
+  d.norm_motion = reinterpret_cast<const uint2*>(nm);
+  d.gi_in = gi_in;
+  d.gi_out = gi_out;
+  d.di_in = di_in;
+  d.di_out = di_out;
+  d.sources = sources;
+  return ARC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int arc_restir_temporal(arc_ctx* c, const arc_restir_params* p, const arc_trace_params* sun_trace,
+                        const arc_trace_params* gi_trace, const uint32_t* d_depth_mat, const uint32_t* d_norm_motion,
+                        const arc_gi_reservoir* d_gi_in, arc_gi_reservoir* d_gi_out, const arc_di_reservoir* d_di_in,
+                        arc_di_reservoir* d_di_out, const arc_emissive_source* d_sources,
+                        const arc_alias_entry* d_alias, const arc_sharc* sharc, const arc_rigid_body* bodies,
+                        uint32_t n_bodies, void* stream) {
+  if (!sun_trace || !gi_trace) return fail(c, ARC_ERR_INVALID, "null trace parameters");
+  if (p && p->n_emissive && !d_alias) return fail(c, ARC_ERR_INVALID, "n_emissive > 0 needs the alias table");
+  if (n_bodies && !bodies) return fail(c, ARC_ERR_INVALID, "null rigid bodies");
+  arcdev::RestirDev d{};
+  int rc = restir_common(c, p, d_depth_mat, d_norm_motion, d_gi_in, d_gi_out, d_di_in, d_di_out, d_sources, d);
+  if (rc != ARC_OK) return rc;
+  d.alias = d_alias;
+  d.has_sharc = sharc ? 1u : 0u;
+  d.n_bodies = std::min<uint32_t>(n_bodies, ARC_MAX_RIGID_BODIES);
+  const hipStream_t s = (hipStream_t)stream;
+  const uint64_t n = (uint64_t)p->width * p->height;
+  HIP_TRY(c, arcdev::launch_rt_setup(d, s));
+  // sun shadow (:149-159): trace_world_64tree(.., 1, true) or trace_shadow_vis
+  arc_shadow_params sun{ARC_SHADOW_SUN, p->shadow_max_steps};
+  if (p->shadow_mode == 0u) {
+    arc_trace_params tp = *sun_trace;
+    tp.semantics = ARC_SEM_WORLD;
+    tp.lod_bias = 1u;
+    tp.ignore_water = 1u;
+    rc = arc_trace_device(c, &tp, d.rays_a, n, const_cast<arc_hit*>(d.hits_a), stream);
+  } else {
+    rc = arc_trace_shadow_device(c, &sun, d.rays_a, n, const_cast<float*>(d.vis_a), nullptr, stream);
+  }
+  if (rc != ARC_OK) return rc;
+  // GI ray (:207): trace_world_64tree(gi_origin, bounce_dir, gi_max_dist, 2, true)
+  arc_trace_params gp = *gi_trace;
+  gp.semantics = ARC_SEM_WORLD;
+  gp.lod_bias = 2u;
+  gp.ignore_water = 1u;
+  rc = arc_trace_full_device(c, &gp, d.rays_b, n, const_cast<arc_trace_result*>(d.full_b), stream);
+  if (rc != ARC_OK) return rc;
+  if (d.n_bodies) {  // :208-215
+    HIP_TRY(c, arcdev::launch_rt_rigid_rays(d, s));
+    rc = arc_trace_rigid_device(c, bodies, d.n_bodies, d.rays_c, n, const_cast<arc_rigid_hit*>(d.rb_c), stream);
+    if (rc != ARC_OK) return rc;
+  }
+  HIP_TRY(c, arcdev::launch_rt_bounce(d, s));
+  if (sharc) {
+    rc = arc_sharc_query(sharc, d.probes, n, p->cam.eye, reinterpret_cast<float*>(const_cast<float4*>(d.sharc_out)),
+                         stream);
+    if (rc != ARC_OK) return fail(c, rc, "SHARC query failed");
+  }
+  rc = arc_trace_shadow_device(c, &sun, d.rays_d, n, const_cast<float*>(d.vis_d), nullptr, stream);
+  if (rc != ARC_OK) return rc;
+  HIP_TRY(c, arcdev::launch_rt_resolve(d, s));
+  return ARC_OK;
+}
+
+int arc_restir_spatial(arc_ctx* c, const arc_restir_params* p, const uint32_t* d_depth_mat,
+                       const uint32_t* d_norm_motion, const arc_gi_reservoir* d_gi_in, arc_gi_reservoir* d_gi_out,
+                       const arc_di_reservoir* d_di_in, arc_di_reservoir* d_di_out,
+                       const arc_emissive_source* d_sources, void* stream) {
+  arcdev::RestirDev d{};
+  int rc = restir_common(c, p, d_depth_mat, d_norm_motion, d_gi_in, d_gi_out, d_di_in, d_di_out, d_sources, d);
+  if (rc != ARC_OK) return rc;
+  const hipStream_t s = (hipStream_t)stream;
+  const uint64_t n = (uint64_t)p->width * p->height;
+  HIP_TRY(c, arcdev::launch_rs_rays(d, s));
+  const arc_shadow_params sun{ARC_SHADOW_SUN, p->shadow_max_steps}, di{ARC_SHADOW_DI, p->shadow_max_steps};
+  rc = arc_trace_shadow_device(c, &sun, d.rays_a, n, const_cast<float*>(d.vis_a), nullptr, stream);
+  if (rc != ARC_OK) return rc;
+  rc = arc_trace_shadow_device(c, &di, d.rays_b, n, const_cast<float*>(d.vis_b), nullptr, stream);
+  if (rc != ARC_OK) return rc;
+  HIP_TRY(c, arcdev::launch_rs_resolve(d, s));
+  return ARC_OK;
 }
 
 }  // extern "C"

@@ -64,6 +64,12 @@ SHADOW_REC_DTYPE = np.dtype([("t_voxels", "<f4"), ("steps", "<u4"), ("vx", "<u2"
 RIGID_BODY_DTYPE = np.dtype([("world_to_local", "<f4", (16,)), ("local_to_world", "<f4", (16,)),
                              ("prev_local_to_world", "<f4", (16,)), ("local_bounds_and_root", "<f4", (4,)),
                              ("world_sphere", "<f4", (4,))])
+GI_RESERVOIR_DTYPE = np.dtype([("sample_pos", "<f4", (3,)), ("norm_m_flags", "<u4"), ("lo_rg", "<u4"),
+                               ("lo_b_w", "<u4"), ("primary_depth", "<f4"), ("shadow_vis_m", "<u4")])
+DI_RESERVOIR_DTYPE = np.dtype([("light_idx_seed", "<u4"), ("w_sum", "<f4"), ("m", "<f4"), ("w", "<f4")])
+EMISSIVE_DTYPE = np.dtype([("pos", "<f4", (3,)), ("radius", "<f4"), ("intensity", "<f4", (3,)),
+                           ("normal_packed", "<u4")])
+ALIAS_DTYPE = np.dtype([("prob", "<f4"), ("alias_idx", "<u4")])
 ARC_MAX_RIGID_BODIES = 32  # trace_rigid_bodies reads min(count, 32) instances
 RIGID_HIT_DTYPE = np.dtype([("t_voxels", "<f4"), ("hit_pos_m", "<f4", (3,)), ("prev_hit_pos_m", "<f4", (3,)),
                             ("normal", "<f4", (3,)), ("contact_ao", "<f4"), ("mat", "<u4"), ("voxel_hash", "<u4"),
@@ -133,6 +139,35 @@ class GenParams(C.Structure):
 class Camera(C.Structure):
     _fields_ = [("eye", C.c_float * 3), ("forward", C.c_float * 3), ("right", C.c_float * 3), ("up", C.c_float * 3),
                 ("tan_half_fov", C.c_float), ("aspect", C.c_float), ("jitter", C.c_float * 2)]
+
+
+class RestirParams(C.Structure):
+    """arc_restir_params: the CameraUniforms / EmissiveParams fields the ReSTIR passes read."""
+    _fields_ = [("cam", Camera), ("prev_eye", C.c_float * 3), ("frame_index", C.c_uint32),
+                ("sun_dir", C.c_float * 3), ("sun_intensity", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("gi_max_dist", C.c_float), ("shadow_max_dist", C.c_float),
+                ("shadow_jitter", C.c_float), ("shadow_mode", C.c_uint32), ("shadow_max_steps", C.c_uint32),
+                ("denoiser_taps", C.c_uint32), ("target_valid", C.c_uint32), ("n_emissive", C.c_uint32),
+                ("total_flux", C.c_float), ("reserved", C.c_uint32)]
+
+    @classmethod
+    def make(cls, cam: "Camera", width: int, height: int, sun_dir=(0.35, 0.8, 0.45), sun_intensity: float = 3.0,
+             frame: int = 0, prev_eye=None, shadow_mode: int = 1, denoiser_taps: int = 16, n_emissive: int = 0,
+             total_flux: float = 0.0, gi_max_dist: float = 26.0, shadow_max_dist: float = 88.0,
+             shadow_jitter: float = 0.018, shadow_max_steps: int = 72) -> "RestirParams":
+        """The high-quality settings of quality.rs:101-134 by default."""
+        p = cls()
+        p.cam = cam
+        p.prev_eye[:] = tuple(cam.eye) if prev_eye is None else tuple(prev_eye)
+        p.frame_index = frame
+        sd = np.asarray(sun_dir, np.float64)
+        p.sun_dir[:] = tuple((sd / np.linalg.norm(sd)).astype(np.float32))
+        p.sun_intensity = sun_intensity
+        p.width, p.height = width, height
+        p.gi_max_dist, p.shadow_max_dist, p.shadow_jitter = gi_max_dist, shadow_max_dist, shadow_jitter
+        p.shadow_mode, p.shadow_max_steps, p.denoiser_taps = shadow_mode, shadow_max_steps, denoiser_taps
+        p.n_emissive, p.total_flux = n_emissive, total_flux
+        return p
 
 
 class ArcView(C.Structure):
@@ -327,6 +362,11 @@ _SIGS = [
     ("arc_snapshot_last_error", C.c_char_p, []),
     ("arc_snapshot_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_int]),
     ("arc_crc32", C.c_uint32, [C.c_void_p, C.c_uint64, C.c_uint32]),
+    ("arc_restir_temporal", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    ("arc_restir_spatial", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("arc_gen_bounce_rays", C.c_int64, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_float, C.c_void_p,
                                         C.c_void_p]),
 ]
@@ -739,6 +779,25 @@ class Context:
 
     def upload_world(self, world: HostWorld, **kw):
         self.upload(*world.pools(), **kw)
+
+    def restir_temporal(self, p: RestirParams, sun_trace: TraceParams, gi_trace: TraceParams, d_depth_mat: int,
+                        d_norm_motion: int, d_gi_in: int, d_gi_out: int, d_di_in: int, d_di_out: int,
+                        d_sources: int = 0, d_alias: int = 0, sharc=None, bodies=None, stream: int = 0) -> None:
+        """Pass 2A (arc_restir_temporal) on device buffers; `sharc` a Sharc or None, `bodies` host
+        RIGID_BODY_DTYPE instances or None."""
+        b = None if bodies is None or len(bodies) == 0 else np.ascontiguousarray(bodies, RIGID_BODY_DTYPE)
+        self._check(lib().arc_restir_temporal(self._h, C.byref(p), C.byref(sun_trace), C.byref(gi_trace), d_depth_mat,
+                                              d_norm_motion, d_gi_in, d_gi_out, d_di_in, d_di_out, d_sources or None,
+                                              d_alias or None, sharc._h if sharc is not None else None,
+                                              b.ctypes.data if b is not None else None, 0 if b is None else len(b),
+                                              stream or None), "arc_restir_temporal")
+
+    def restir_spatial(self, p: RestirParams, d_depth_mat: int, d_norm_motion: int, d_gi_in: int, d_gi_out: int,
+                       d_di_in: int, d_di_out: int, d_sources: int = 0, stream: int = 0) -> None:
+        """Pass 2B (arc_restir_spatial) on device buffers."""
+        self._check(lib().arc_restir_spatial(self._h, C.byref(p), d_depth_mat, d_norm_motion, d_gi_in, d_gi_out,
+                                             d_di_in, d_di_out, d_sources or None, stream or None),
+                    "arc_restir_spatial")
 
     def upload_snapshot(self, snap: Snapshot, node_capacity: int = 0, brick_capacity: int = 0,
                         nthreads: int = 8) -> None:

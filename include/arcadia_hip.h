@@ -495,6 +495,78 @@ int arc_batch_get_launch_config(const arc_batch* b, arc_launch_config* cfg);
 int arc_batch_autotune(arc_batch* b, arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
                        float* best_ms);
 
+/* ---- ReSTIR GI / DI passes (shaders/lighting_restir_gi.wgsl) ------------------------------- */
+/* GiReservoir (types.wgsl:90-106; crates/arcadia-render/src/gbuffer.rs), 32 B:
+ *   norm_m_flags = oct snorm8x2 secondary normal | M << 16 | sample_age << 24 | is_sky << 31,
+ *   lo_rg / lo_b_w = f16 pairs (Lo.r, Lo.g) / (Lo.b, W), shadow_vis_m = f16 pair (sun vis, count). */
+typedef struct arc_gi_reservoir {
+  float sample_pos[3];
+  uint32_t norm_m_flags;
+  uint32_t lo_rg;
+  uint32_t lo_b_w;
+  float primary_depth;
+  uint32_t shadow_vis_m;
+} arc_gi_reservoir;
+/* DiReservoir (shaders/common/di.wgsl:29-35): light index | seed << 16, w_sum, M, W. 16 B. */
+typedef struct arc_di_reservoir {
+  uint32_t light_idx_seed;
+  float w_sum;
+  float m;
+  float w;
+} arc_di_reservoir;
+/* EmissiveSource (di.wgsl:7-12; crates/arcadia-world/src/emissive_registry.rs), 32 B, and its
+ * alias-table entry (di.wgsl:15-18), 8 B: produced by the caller's emissive registry. */
+typedef struct arc_emissive_source {
+  float pos[3];
+  float radius;
+  float intensity[3];
+  uint32_t normal_packed; /* 10-bit x | y << 10 | z << 20, each (n + 1) / 2 * 1023 */
+} arc_emissive_source;
+typedef struct arc_alias_entry {
+  float prob;
+  uint32_t alias_idx;
+} arc_alias_entry;
+/* The CameraUniforms / EmissiveParams fields the passes read (types.wgsl:19-63, di.wgsl:21-26). */
+typedef struct arc_restir_params {
+  arc_camera cam;            /* eye, forward, right, up, tan_half_fov, aspect, jitter_brush_dt.xy */
+  float prev_eye[3];         /* prev_cam_pos.xyz */
+  uint32_t frame_index;      /* bitcast(cam_up_frame.w) */
+  float sun_dir[3];          /* sun_dir_intensity.xyz (towards the sun, unit) */
+  float sun_intensity;       /* sun_dir_intensity.w */
+  uint32_t width, height;    /* world_grid_dims.w */
+  float gi_max_dist;         /* ray_distances.z (m) */
+  float shadow_max_dist;     /* ray_distances.w (m) */
+  float shadow_jitter;       /* lod_settings.w */
+  uint32_t shadow_mode;      /* quality_flags.z: 0 = 64-tree trace, 1 = any-hit sun walker */
+  uint32_t shadow_max_steps; /* ray_steps.w: step budget of the any-hit walkers */
+  uint32_t denoiser_taps;    /* quality_flags.w & 0xFF (<= 12: checkerboard half-rate mode) */
+  uint32_t target_valid;     /* target_voxel_info.w > 0.5 */
+  uint32_t n_emissive;       /* EmissiveParams.count (<= 16384) */
+  float total_flux;          /* EmissiveParams.total_flux */
+  uint32_t reserved;
+} arc_restir_params;
+/* Pass 2A, temporal_main (lighting_restir_gi.wgsl:62-455): sun shadow with its temporal average,
+ * one GI candidate per pixel (cosine bounce, emissive macro-cell guiding, world + rigid-body
+ * hit, SHARC or bounce-sun radiance, sky), RIS merge with the reprojected history, and the DI
+ * candidates + temporal reuse. G-buffer: d_depth_mat / d_norm_motion as arc_primary_gbuffer writes
+ * them (2 u32 per pixel each, row-major). Reservoirs: width * height each; *_in is the previous
+ * frame's spatial output. sun_trace / gi_trace: traversal parameters of the lod_bias 1 sun trace
+ * (shadow_mode 0) and the lod_bias 2 GI trace (ignore_water set by the pass). sharc may be NULL
+ * (every lookup misses); d_sources / d_alias may be NULL when n_emissive is 0; bodies are host
+ * instances (n_bodies <= 32 used). Asynchronous on `stream`. */
+int arc_restir_temporal(arc_ctx* ctx, const arc_restir_params* p, const arc_trace_params* sun_trace,
+                        const arc_trace_params* gi_trace, const uint32_t* d_depth_mat, const uint32_t* d_norm_motion,
+                        const arc_gi_reservoir* d_gi_in, arc_gi_reservoir* d_gi_out,
+                        const arc_di_reservoir* d_di_in, arc_di_reservoir* d_di_out,
+                        const arc_emissive_source* d_sources, const arc_alias_entry* d_alias,
+                        const arc_sharc* sharc, const arc_rigid_body* bodies, uint32_t n_bodies, void* stream);
+/* Pass 2B, spatial_main (:457-762): 2-3 rotated neighbour taps with depth / normal rejection,
+ * Jacobian-weighted GI reconnection with a visibility ray for a winning neighbour, DI neighbour
+ * reuse and the DI visibility ray. *_in = the temporal pass's output. Asynchronous. */
+int arc_restir_spatial(arc_ctx* ctx, const arc_restir_params* p, const uint32_t* d_depth_mat,
+                       const uint32_t* d_norm_motion, const arc_gi_reservoir* d_gi_in, arc_gi_reservoir* d_gi_out,
+                       const arc_di_reservoir* d_di_in, arc_di_reservoir* d_di_out,
+                       const arc_emissive_source* d_sources, void* stream);
 
 #ifdef __cplusplus
 }

@@ -817,6 +817,22 @@ void orc_trace_rigid(const orc_world* w, const arc_rigid_body* bodies, uint32_t 
   for (uint64_t i = 0; i < n; ++i) rigid_all(w, bodies, n_bodies, &rays[i], &out[i]);
 }
 
+/* Single-ray forms of the traversals, for restatements that trace inline (restir_oracle.c). */
+void orc_trace_world_ray(const orc_world* w, const arc_trace_params* p, const arc_ray* ray, arc_trace_result* res) {
+  orc_counters c;
+  trace_world_one(w, p, ray, res, &c);
+}
+float orc_shadow_ray(const orc_world* w, uint32_t mode, uint32_t ray_steps_w, const arc_ray* ray) {
+  float vis;
+  arc_shadow_rec rec;
+  shadow_one(w, mode, ray_steps_w, ray, &vis, &rec);
+  return vis;
+}
+void orc_rigid_ray(const orc_world* w, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* ray,
+                   arc_rigid_hit* out) {
+  rigid_all(w, bodies, n_bodies, ray, out);
+}
+
 /* query_voxel_world_meters (traversal64.wgsl:182-220) */
 uint32_t orc_query_point(const orc_world* w, const float p[3]) {
   float vp[3];

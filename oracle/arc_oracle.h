@@ -13,6 +13,7 @@
  *   orc_trace_shadow  trace_shadow_vis / _di    shaders/common/traversal64.wgsl:571-773 / 781-989
  *   orc_trace_rigid   trace_rigid_bodies        shaders/common/traversal64.wgsl:1005-1244
  *   orc_query_point   query_voxel_world_meters  shaders/common/traversal64.wgsl:182-220
+ *   orc_restir_*      temporal_main / spatial_main  shaders/lighting_restir_gi.wgsl:62-762 (restir_oracle.c)
  * plus an independent checker that shares no code with the above:
  *   orc_brute_dense   Amanatides-Woo unit-voxel walk in double precision over a dense grid.
  *
@@ -71,6 +72,36 @@ void orc_trace_shadow(const orc_world* w, const arc_shadow_params* sp, const arc
 void orc_trace_rigid(const orc_world* w, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* rays,
                      uint64_t n, arc_rigid_hit* out);
 uint32_t orc_query_point(const orc_world* w, const float p[3]);
+/* One ray of orc_trace_world / orc_trace_shadow / orc_trace_rigid. */
+void orc_trace_world_ray(const orc_world* w, const arc_trace_params* p, const arc_ray* ray, arc_trace_result* res);
+float orc_shadow_ray(const orc_world* w, uint32_t mode, uint32_t ray_steps_w, const arc_ray* ray);
+void orc_rigid_ray(const orc_world* w, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* ray,
+                   arc_rigid_hit* out);
+
+/* ReSTIR GI / DI passes, temporal_main and spatial_main of shaders/lighting_restir_gi.wgsl
+ * (:62-455, :457-762), one pixel at a time with the traversals inline, as the shader is written
+ * (restir_oracle.c). The material palette (256 x {albedo rgb, roughness, emission rgb, variance})
+ * and the SHARC table are inputs; sharc_resolved NULL = no SHARC (every lookup misses). */
+typedef struct orc_restir_in {
+  const arc_restir_params* p;
+  const arc_trace_params* sun_trace;
+  const arc_trace_params* gi_trace;
+  const uint32_t* depth_mat;
+  const uint32_t* norm_motion;
+  const arc_gi_reservoir* gi_in;
+  const arc_di_reservoir* di_in;
+  const arc_emissive_source* sources;
+  const arc_alias_entry* alias;
+  const float* palette;
+  uint32_t sharc_log2_buckets;
+  const uint32_t* sharc_resolved;
+  const arc_rigid_body* bodies;
+  uint32_t n_bodies;
+} orc_restir_in;
+void orc_restir_temporal(const orc_world* w, const orc_restir_in* in, arc_gi_reservoir* gi_out,
+                         arc_di_reservoir* di_out, int nthreads);
+void orc_restir_spatial(const orc_world* w, const orc_restir_in* in, arc_gi_reservoir* gi_out,
+                        arc_di_reservoir* di_out, int nthreads);
 /* Compact records as the kernels write them. */
 void orc_compact(const arc_trace_result* full, uint64_t n, int world_units, arc_hit* out);
 

@@ -280,3 +280,73 @@ class Sharc:
             lib().orc_sharc_query(self.log2, self.resolved.ctypes.data, q.ctypes.data, len(q), (C.c_float * 3)(*cam),
                                   out.ctypes.data)
         return out
+
+
+# ---- ReSTIR GI / DI passes (restir_oracle.c) ----------------------------------------------------
+GI_RESERVOIR_DTYPE = np.dtype([("sample_pos", "<f4", (3,)), ("norm_m_flags", "<u4"), ("lo_rg", "<u4"),
+                               ("lo_b_w", "<u4"), ("primary_depth", "<f4"), ("shadow_vis_m", "<u4")])
+DI_RESERVOIR_DTYPE = np.dtype([("light_idx_seed", "<u4"), ("w_sum", "<f4"), ("m", "<f4"), ("w", "<f4")])
+
+
+class _RestirIn(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("sun_trace", C.c_void_p), ("gi_trace", C.c_void_p), ("depth_mat", C.c_void_p),
+                ("norm_motion", C.c_void_p), ("gi_in", C.c_void_p), ("di_in", C.c_void_p), ("sources", C.c_void_p),
+                ("alias", C.c_void_p), ("palette", C.c_void_p), ("sharc_log2_buckets", C.c_uint32),
+                ("sharc_resolved", C.c_void_p), ("bodies", C.c_void_p), ("n_bodies", C.c_uint32)]
+
+
+def _restir(fn, world: World, params, depth_mat, norm_motion, gi_in, di_in, palette, sun_trace=None, gi_trace=None,
+            sources=None, alias=None, sharc=None, bodies=None, nthreads: int = 0):
+    n = params.width * params.height
+    keep = []
+
+    def arr(a, dtype=None):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a if dtype is None else np.asarray(a).view(dtype) if np.asarray(a).dtype != dtype
+                                 else a)
+        keep.append(a)
+        return a.ctypes.data if a.size else None
+
+    inp = _RestirIn()
+    inp.p = C.cast(C.pointer(params), C.c_void_p)
+    keep.append(params)
+    if sun_trace is not None:
+        inp.sun_trace = C.cast(C.pointer(sun_trace), C.c_void_p)
+    if gi_trace is not None:
+        inp.gi_trace = C.cast(C.pointer(gi_trace), C.c_void_p)
+    inp.depth_mat = arr(np.asarray(depth_mat, np.uint32))
+    inp.norm_motion = arr(np.asarray(norm_motion, np.uint32))
+    inp.gi_in = arr(np.asarray(gi_in).view(GI_RESERVOIR_DTYPE))
+    inp.di_in = arr(np.asarray(di_in).view(DI_RESERVOIR_DTYPE))
+    inp.sources = arr(sources)
+    inp.alias = arr(alias)
+    inp.palette = arr(np.asarray(palette, np.float32))
+    if sharc is not None:
+        inp.sharc_log2_buckets = sharc[0]
+        inp.sharc_resolved = arr(np.asarray(sharc[1], np.uint32))
+    if bodies is not None and len(bodies):
+        inp.bodies = arr(np.ascontiguousarray(bodies, RIGID_BODY_DTYPE))
+        inp.n_bodies = len(bodies)
+    gi_out = np.zeros(n, GI_RESERVOIR_DTYPE)
+    di_out = np.zeros(n, DI_RESERVOIR_DTYPE)
+    f = getattr(lib(), fn)
+    f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    f.restype = None
+    f(world.ref, C.byref(inp), gi_out.ctypes.data, di_out.ctypes.data, _threads(nthreads))
+    return gi_out, di_out
+
+
+def restir_temporal(world: World, params, sun_trace, gi_trace, depth_mat, norm_motion, gi_in, di_in, palette,
+                    sources=None, alias=None, sharc=None, bodies=None, nthreads: int = 0):
+    """temporal_main over every pixel -> (GI reservoirs, DI reservoirs). sharc = (log2_buckets,
+    resolved u32 table) or None; palette = 256 x 8 f32."""
+    return _restir("orc_restir_temporal", world, params, depth_mat, norm_motion, gi_in, di_in, palette, sun_trace,
+                   gi_trace, sources, alias, sharc, bodies, nthreads)
+
+
+def restir_spatial(world: World, params, depth_mat, norm_motion, gi_in, di_in, palette, sources=None,
+                   nthreads: int = 0):
+    """spatial_main over every pixel -> (GI reservoirs, DI reservoirs)."""
+    return _restir("orc_restir_spatial", world, params, depth_mat, norm_motion, gi_in, di_in, palette,
+                   sources=sources, nthreads=nthreads)
