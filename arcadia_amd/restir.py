@@ -64,3 +64,17 @@ def emissive_sources(pos, radius, intensity, normals=None, eta=None) -> np.ndarr
     et = np.broadcast_to(np.asarray(1.0 if eta is None else eta, np.float64), (n,))
     s["normal_packed"] = [pack_emitter_normal(nrm[i], et[i]) for i in range(n)]
     return s
+
+
+def view_proj(cam) -> np.ndarray:
+    """Column-major 4x4 (glam to_cols_array order) whose clip.xy / clip.w is the NDC of the camera's
+    pinhole model: the prev_view_proj that gives zero motion for a camera that did not move."""
+    r, u, f, e = (np.array(list(getattr(cam, k)), np.float64) for k in ("right", "up", "forward", "eye"))
+    m = np.zeros((4, 4))
+    m[0, :3] = r / (cam.aspect * cam.tan_half_fov)
+    m[0, 3] = -r @ e / (cam.aspect * cam.tan_half_fov)
+    m[1, :3] = u / cam.tan_half_fov
+    m[1, 3] = -u @ e / cam.tan_half_fov
+    m[3, :3] = f
+    m[3, 3] = -f @ e
+    return m.T.reshape(-1).astype(np.float32)

@@ -358,6 +358,91 @@ def snapshot_leg(args, ctx, world, pools, threads):
     return res
 
 
+def restir_leg(args, ctx, pools, prays, phits, stream, threads):
+    """ReSTIR GI / DI passes at 1080p on the full-frame world: G-buffer of the still camera, then
+    temporal + spatial per frame with the history carried over, 64 emissive sources 0.6 m above
+    visible surfaces (DI), no SHARC (every GI hit takes the bounce-sun shadow ray), 72-step any-hit
+    shadows, the high-quality settings. Frames 2.. are timed with HIP events; frame 0 is checked
+    against the oracle over every pixel."""
+    import torch
+
+    import arcadia_amd as A
+    from arcadia_amd import restir as R
+    from arcadia_amd import shadow_rays as S
+
+    W, H = 1920, 1080
+    cam = A.camera_look((31.3, 58.0, 27.1), 0.7, -0.5, 65.0, W, H)
+    rng = np.random.default_rng(args.seed + 11)
+    _, hp, hn = S.hit_points(prays, phits)
+    pick = rng.choice(len(hp), 64, replace=False)
+    src = R.emissive_sources((hp[pick] + hn[pick] * 0.6).astype(np.float32), 0.25, rng.uniform(1.0, 6.0, (64, 3)),
+                             normals=hn[pick], eta=rng.uniform(0.0, 1.0, 64))
+    lum = src["intensity"].astype(np.float64) @ np.array([0.2126, 0.7152, 0.0722])
+    alias = R.alias_table(lum)
+    n = W * H
+    g = A.GbufferParams.from_camera(cam, W, H, prev_view_proj=R.view_proj(cam))
+    d_dm = torch.zeros(n * 2, dtype=torch.int32, device="cuda")
+    d_nm = torch.zeros(n * 2, dtype=torch.int32, device="cuda")
+    d_aa = torch.zeros(n, dtype=torch.int32, device="cuda")
+    sptr = stream.cuda_stream
+    ctx.primary_gbuffer(A.TraceParams.exact(0), g, d_dm.data_ptr(), d_nm.data_ptr(), d_aa.data_ptr(), sptr)
+    gi = [torch.zeros(n * 32, dtype=torch.uint8, device="cuda") for _ in range(3)]
+    di = [torch.zeros(n * 16, dtype=torch.uint8, device="cuda") for _ in range(3)]
+    gi[0].view(torch.float32).view(-1, 8)[:, 6] = 1e5  # empty history: primary_depth 1e5
+    di[0].view(torch.int32).view(-1, 4)[:, 0] = -1      # light_idx_seed 0xFFFFFFFF
+    d_src = torch.from_numpy(src.view(np.uint8).copy()).cuda()
+    d_al = torch.from_numpy(alias.view(np.uint8).copy()).cuda()
+    sun_t, gi_t = A.TraceParams.exact(1, True), A.TraceParams.exact(2, True)
+    k = max(4, min(args.steps, 10))
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(k)]
+    hist = 0  # gi[hist] / di[hist] = the history the next temporal pass reads
+    frame0 = None
+    for f in range(k):
+        p = A.RestirParams.make(cam, W, H, frame=f, n_emissive=64, total_flux=float(lum.sum()), sun_intensity=3.0)
+        tmp, out = (hist + 1) % 3, (hist + 2) % 3
+        ev[f][0].record(stream)
+        ctx.restir_temporal(p, sun_t, gi_t, d_dm.data_ptr(), d_nm.data_ptr(), gi[hist].data_ptr(), gi[tmp].data_ptr(),
+                            di[hist].data_ptr(), di[tmp].data_ptr(), d_src.data_ptr(), d_al.data_ptr(), stream=sptr)
+        ev[f][1].record(stream)
+        ctx.restir_spatial(p, d_dm.data_ptr(), d_nm.data_ptr(), gi[tmp].data_ptr(), gi[out].data_ptr(),
+                           di[tmp].data_ptr(), di[out].data_ptr(), d_src.data_ptr(), stream=sptr)
+        ev[f][2].record(stream)
+        if f == 0:
+            torch.cuda.synchronize()
+            frame0 = (p, [x.cpu().numpy() for x in (d_dm, d_nm, gi[hist], di[hist], gi[tmp], di[tmp], gi[out], di[out])])
+        hist = out
+    torch.cuda.synchronize()
+    t_ms = [ev[f][0].elapsed_time(ev[f][1]) for f in range(2, k)]
+    s_ms = [ev[f][1].elapsed_time(ev[f][2]) for f in range(2, k)]
+    go = gi[hist].cpu().numpy().view(A.GI_RESERVOIR_DTYPE)
+    exact = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        p, (dm, nm, gi0, di0, gt, dt, g1, d1) = frame0
+        dm, nm = dm.view(np.uint32).reshape(-1, 2), nm.view(np.uint32).reshape(-1, 2)
+        ow = O.World(*pools)
+        ot, odt = O.restir_temporal(ow, p, sun_t, gi_t, dm, nm, gi0.view(A.GI_RESERVOIR_DTYPE),
+                                    di0.view(A.DI_RESERVOIR_DTYPE), R.palette(), sources=src, alias=alias,
+                                    nthreads=threads)
+        oo, odo = O.restir_spatial(ow, p, dm, nm, ot, odt, R.palette(), sources=src, nthreads=threads)
+        exact = bool(all(np.array_equal(a.view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+                         for a, b in ((gt, ot), (dt, odt), (g1, oo), (d1, odo))))
+    res = {
+        "metric": "ms per 1080p frame, ReSTIR GI/DI temporal + spatial (full-frame world, 64 emitters, no SHARC)",
+        "temporal_ms": round(float(np.median(t_ms)), 4),
+        "spatial_ms": round(float(np.median(s_ms)), 4),
+        "frames_timed": len(t_ms),
+        "pixels": n,
+        "mean_M": round(float(((go["norm_m_flags"] >> 16) & 0xFF).mean()), 3),
+        "frame0_oracle_exact": exact,
+    }
+    print(f"[bench] restir: temporal {res['temporal_ms']} ms, spatial {res['spatial_ms']} ms per 1080p frame, "
+          f"mean M {res['mean_M']}, frame 0 exact {exact}", file=sys.stderr, flush=True)
+    return res
+
+
 def cache_leg(args, threads):
     """Learned radiance cache vs SHARC (2^22 entries) vs no cache, rel-MSE against the FP64
     1024-spp path-traced judge at 1920x1080 on the full-frame world with 8 emissive clusters."""
@@ -536,6 +621,7 @@ def main() -> int:
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-walker workload")
     ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-body workload")
     ap.add_argument("--no-snapshot", action="store_true", help="skip the world-snapshot workload")
+    ap.add_argument("--no-restir", action="store_true", help="skip the ReSTIR pass workload")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache workload")
     ap.add_argument("--cache-ref-spp", type=int, default=1024)
     ap.add_argument("--cache-frames", type=int, default=60)
@@ -554,6 +640,7 @@ def main() -> int:
         args.no_shadow = True
         args.no_rigid = True
         args.no_snapshot = True
+        args.no_restir = True
         args.no_cache = True
         args.no_destruction = True
 
@@ -688,6 +775,8 @@ def main() -> int:
         out["rigid"] = rigid_leg(args, ctx, pools, prays, phits, stream, threads)
     if not args.no_snapshot:
         out["snapshot"] = snapshot_leg(args, ctx, world, pools, threads)
+    if not args.no_restir:
+        out["restir"] = restir_leg(args, ctx, pools, prays, phits, stream, threads)
     ctx.close()
     del d_prays, d_phits, d_brays, d_bhits
     if not args.no_batched:

@@ -38,13 +38,14 @@ def test_header_struct_sizes_match_bindings(tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "arcadia_hip.h"\n#include "arcadia_world.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(arc_node64), "
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\", sizeof(arc_node64), "
         "sizeof(arc_brick64), sizeof(arc_ray), sizeof(arc_hit), sizeof(arc_trace_result), "
         "sizeof(arc_trace_params), sizeof(arc_gen_params), sizeof(arc_camera), sizeof(arc_gpu_patch), "
         "offsetof(arc_hit, steps), sizeof(arc_gbuffer_params), offsetof(arc_gbuffer_params, width), "
         "sizeof(arc_shadow_params), sizeof(arc_shadow_rec), offsetof(arc_shadow_rec, outcome), "
         "sizeof(arc_rigid_body), offsetof(arc_rigid_body, world_sphere), sizeof(arc_rigid_hit), "
-        "offsetof(arc_rigid_hit, body));return 0;}\n")
+        "offsetof(arc_rigid_hit, body), sizeof(arc_gi_reservoir), sizeof(arc_di_reservoir), "
+        "sizeof(arc_emissive_source), sizeof(arc_alias_entry), sizeof(arc_restir_params));return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
@@ -55,7 +56,9 @@ def test_header_struct_sizes_match_bindings(tmp_path):
             native.GbufferParams.width.offset, ctypes.sizeof(native.ShadowParams), native.SHADOW_REC_DTYPE.itemsize,
             native.SHADOW_REC_DTYPE.fields["outcome"][1], native.RIGID_BODY_DTYPE.itemsize,
             native.RIGID_BODY_DTYPE.fields["world_sphere"][1], native.RIGID_HIT_DTYPE.itemsize,
-            native.RIGID_HIT_DTYPE.fields["body"][1]]
+            native.RIGID_HIT_DTYPE.fields["body"][1], native.GI_RESERVOIR_DTYPE.itemsize,
+            native.DI_RESERVOIR_DTYPE.itemsize, native.EMISSIVE_DTYPE.itemsize, native.ALIAS_DTYPE.itemsize,
+            ctypes.sizeof(native.RestirParams)]
     assert got == want
 
 
