@@ -87,13 +87,14 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
 
 static_assert(32u + arcdev::kQueueWords <= 64u, "queue words");
 
-// Traversal kernel, measured on MI355X for primary and diffuse rays: 6 workgroups of 256 per CU
-// (6 waves per SIMD; the occupancy limit of 8 is slower), refill at 16 idle lanes.
+// Traversal kernel, measured on MI355X for primary and diffuse rays (full-frame world, ms per
+// primary + bounce step): 4 workgroups of 256 per CU 1.24, 5: 1.25, 6: 1.27, 3: 1.33, 2: 1.60,
+// 1: 2.68; refill at 16 idle lanes.
 // blocks_per_cu 0 = as many workgroups as the kernel's occupancy keeps resident (the path kernel's
 // default: its register budget allows fewer).
 arc_launch_config default_config() {
   arc_launch_config c;
-  c.blocks_per_cu = 6;
+  c.blocks_per_cu = 4;
   c.block_threads = 256;
   c.refill_lanes = 16;
   c.batch_rays = 64;
