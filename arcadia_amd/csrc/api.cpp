@@ -1123,6 +1123,8 @@ int restir_common(arc_ctx* c, const arc_restir_params* p, const uint32_t* dm, co
   if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "ReSTIR pass before arc_world_upload");
   if (p->width == 0 || p->height == 0 || p->width > 65535 || p->height > 65535)
     return fail(c, ARC_ERR_INVALID, "width and height must be 1..65535 (world_grid_dims.w packs them in 16 bits)");
+  if ((uint64_t)p->width * p->height > 0x7FFFFFFFull)
+    return fail(c, ARC_ERR_INVALID, "at most 2^31-1 pixels per pass");
   if (!dm || !nm || !gi_in || !gi_out || !di_in || !di_out)
     return fail(c, ARC_ERR_INVALID, "null G-buffer or reservoir pointer");
   if (p->n_emissive > 16384u) return fail(c, ARC_ERR_INVALID, "n_emissive above DI_MAX_SOURCES (16384)");

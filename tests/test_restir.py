@@ -413,3 +413,27 @@ def test_gpu_restir_matches_oracle(ctx, case):
     assert (go["norm_m_flags"] >> 16 & 0xFF > 1).mean() > 0.3
     if n_em:
         assert (do["m"] > 0).mean() > 0.3
+
+
+@pytest.mark.gpu
+def test_gpu_restir_argument_errors(ctx):
+    import torch
+
+    w = slab_world()
+    ctx.upload(*w.pools())
+    cam = A.camera_look((16.0, 3.0, 14.0), 0.4, -1.1, 70.0, 32, 18)
+    n = 32 * 18
+    buf = [torch.zeros(n * 32, dtype=torch.uint8, device="cuda") for _ in range(6)]
+    ptr = [b.data_ptr() for b in buf]
+    tp = A.TraceParams.exact(1, True)
+    bad = [params(cam, 32, 18, shadow_mode=2), params(cam, 0, 18), params(cam, 32, 18, n_emissive=4),
+           params(cam, 65535, 65535)]
+    for p in bad:
+        with pytest.raises(A.ArcError) as e:
+            ctx.restir_temporal(p, tp, tp, ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5])
+        assert e.value.code == A.ARC_ERR_INVALID
+        with pytest.raises(A.ArcError):
+            ctx.restir_spatial(p, ptr[0], ptr[1], ptr[2], ptr[3], ptr[4], ptr[5])
+    with pytest.raises(A.ArcError) as e:  # a null G-buffer
+        ctx.restir_temporal(params(cam, 32, 18), tp, tp, 0, ptr[1], ptr[2], ptr[3], ptr[4], ptr[5])
+    assert e.value.code == A.ARC_ERR_INVALID
