@@ -60,10 +60,11 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh (empty chunk: 256)
   bool cand = false;  // set bit at a leaf or at the LOD floor: hit unless the voxel is air/water
   if (descend) {
-    // descent (:390-552), at most four levels, written with a single loop exit (one exec-mask
-    // update per level instead of one per way out)
-    bool go;
-    do {
+    // descent (:390-552): at most three loads (shift 6 -> 4 -> 2 -> 0; shift 0 always stops), as
+    // four unrolled tests whose exit is wave-uniform, so the compiler keeps no per-lane loop-exit
+    // masks. A lane that has stopped repeats its own test with unchanged state until the wave exits.
+#pragma unroll
+    for (uint32_t pass = 0; pass < 4u; ++pass) {
       const uint32_t b = child_bit(L, sh);
       const bool water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
                          (node.w >> 24) == ARC_MAT_WATER;
@@ -74,13 +75,14 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       const bool stop = (node.z & ARC_IS_LEAF_FLAG) != 0u || sh <= minsh;
       if (water) span_mul = 4u;
       cand = has && stop;
-      go = has && !stop;
+      const bool go = has && !stop;
+      if (__builtin_amdgcn_ballot_w64(go) == 0ull) break;
       if (go) {
         node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
         sh -= 2u;
-        if (sh != 0u) *cold_slot(C, sh) = node;
+        *cold_slot(C, sh) = node;
       }
-    } while (go);
+    }
   }
   if (cand) {  // :441-538
     const uint32_t b = child_bit(L, sh);

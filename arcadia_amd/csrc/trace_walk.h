@@ -44,9 +44,11 @@ struct Lane {
 // Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
 //   slot 0: chunk root node (after a hit: the node the hit resolved in),
 //   slot 1: cached level-1 ancestor, slot 2: cached level-2 ancestor (slot of shift s:
-//   (6 - s) / 2; the level-3 ancestor is the lane's `node`).
+//   (6 - s) / 2; the level-3 ancestor is the lane's `node`),
+//   slot 3: receives the level-3 node a descent writes like any other level (never read), so
+//   the store needs no branch.
 constexpr uint32_t kMaxBlock = 256u;
-constexpr uint32_t kColdSlots = 3u;
+constexpr uint32_t kColdSlots = 4u;
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -63,19 +65,17 @@ __device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
 }
 
 // LOD min_shift (traversal64.wgsl:355-370); `sidx` is the iteration index.
+// The three classes as one branch-free test: class 1 reaches shift 4 past lod_t_far or
+// lod_step and shift 2 past lod_t only; classes 0 and 2 reach shift 2 past lod_t or lod_step and
+// never 4 (t > +inf and sidx > 0xFFFFFFFF are false, as is every comparison with a NaN t).
 template <uint32_t SEM>
 __device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, uint32_t sidx) {
   if (SEM == ARC_SEM_CHUNK) return 0u;
-  uint32_t m = 0u;
-  if (p.lod_bias == 0u) {
-    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
-  } else if (p.lod_bias == 1u) {
-    if (t > p.lod_t_far || sidx > p.lod_step) m = 4u;
-    else if (t > p.lod_t) m = 2u;
-  } else {
-    if (t > p.lod_t || sidx > p.lod_step) m = 2u;
-  }
-  return m;
+  const bool c1 = p.lod_bias == 1u;
+  const float t4 = c1 ? p.lod_t_far : __builtin_huge_valf();
+  const uint32_t s4 = c1 ? p.lod_step : 0xFFFFFFFFu;
+  const uint32_t s2 = c1 ? 0xFFFFFFFFu : p.lod_step;
+  return (t > t4 || sidx > s4) ? 4u : ((t > p.lod_t || sidx > s2) ? 2u : 0u);
 }
 
 // Guards at the top of an outer iteration (traversal64.wgsl:302-310 / raycast_cpu.rs:78-84).
@@ -85,7 +85,7 @@ __device__ __forceinline__ bool iteration_guards_fail(const DevWorld& w, const D
   const uint32_t wx = SEM == ARC_SEM_CHUNK ? 256u : (w.gx << 8);
   const uint32_t wy = SEM == ARC_SEM_CHUNK ? 256u : (w.gy << 8);
   const uint32_t wz = SEM == ARC_SEM_CHUNK ? 256u : (w.gz << 8);
-  return L.step >= max_steps || L.t >= L.tl || L.vx >= wx || L.vy >= wy || L.vz >= wz;
+  return (L.step >= max_steps) | (L.t >= L.tl) | (L.vx >= wx) | (L.vy >= wy) | (L.vz >= wz);
 }
 
 // Restart level of an iteration without a chunk change (traversal64.wgsl:374-386). The
@@ -133,9 +133,16 @@ __device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLds& R) {
 // the same) or from the LDS copy; an empty chunk returns false (the caller steps over 256
 // voxels). Otherwise the restart level is restart_shift() and the node the cached ancestor
 // there. On return `sh` is the level the descent starts on and `node` the node at that level.
+// The cached-ancestor read runs for every lane; a chunk change (restart_shift gives 6 there) then
+// replaces the node it read (the previous chunk's root) with the new chunk's root. Two plain
+// branches, the second skipped by whole waves, instead of an if / else.
 template <uint32_t SEM>
 __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4* C, const RootLds& R, bool lds_roots,
                                         uint32_t minsh, uint32_t& sh, uint4& node) {
+  const uint32_t rs = restart_shift(L.diff, minsh, sh);
+  if (rs != sh) node = *cold_slot(C, rs);
+  sh = rs;
+  bool ok = true;
   if (SEM == ARC_SEM_WORLD && L.diff >= 256u) {
     const uint32_t key = (L.vx >> 8) + __umul24(L.vy >> 8, w.gx) + __umul24(L.vz >> 8, w.gx * w.gy);
     uint32_t root;
@@ -146,15 +153,10 @@ __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4*
       root = w.roots[key];
       if (root != ARC_EMPTY_CHUNK_SENTINEL) node = w.nodes[root];
     }
-    sh = 6u;
-    if (root == ARC_EMPTY_CHUNK_SENTINEL) return false;
-    C[0] = node;
-    return true;
+    ok = root != ARC_EMPTY_CHUNK_SENTINEL;
+    if (ok) C[0] = node;
   }
-  const uint32_t rs = restart_shift(L.diff, minsh, sh);
-  if (rs != sh) node = *cold_slot(C, rs);
-  sh = rs;
-  return true;
+  return ok;
 }
 
 // step_dda_box (traversal64.wgsl:152-179 / raycast_cpu.rs:245-292) over the box of `span`
