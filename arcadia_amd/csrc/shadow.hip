@@ -46,22 +46,25 @@ __device__ __forceinline__ void shadow_iteration(const DevWorld& w, const DevPar
   uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh
   bool cand = false;                      // the walk ends here unless the cell is water
   if (descend) {
-    for (;;) {  // :672-755 / :882-971
+    // :672-755 / :882-971, in the traversal's form: at most three loads, four unrolled tests with
+    // a wave-uniform exit; a lane that has stopped repeats its own test until the wave exits.
+    const uint32_t stop_flags = MODE == ARC_SHADOW_SUN ? ARC_IS_LEAF_FLAG : (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG);
+#pragma unroll
+    for (uint32_t pass = 0; pass < 4u; ++pass) {
       const uint32_t b = child_bit(L, sh);
-      if ((node.z & ARC_IS_UNIFORM_FLAG) != 0u && (node.w >> 24) == ARC_MAT_WATER) {  // water fast-skip
-        span_mul = 4u;
-        break;
-      }
+      const bool water = (node.z & ARC_IS_UNIFORM_FLAG) != 0u && (node.w >> 24) == ARC_MAT_WATER;  // fast-skip
       const uint64_t below = (((uint64_t)node.y << 32) | node.x) << (63u - b);
-      if ((int64_t)below >= 0) break;  // no child at b
-      const uint32_t stop_flags = MODE == ARC_SHADOW_SUN ? ARC_IS_LEAF_FLAG : (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG);
-      if ((node.z & stop_flags) != 0u || sh <= minsh) {
-        cand = true;
-        break;
+      const bool has = (int64_t)below < 0 && !water;  // child at b
+      const bool stop = (node.z & stop_flags) != 0u || sh <= minsh;
+      if (water) span_mul = 4u;
+      cand = has && stop;
+      const bool go = has && !stop;
+      if (__builtin_amdgcn_ballot_w64(go) == 0ull) break;
+      if (go) {
+        node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
+        sh -= 2u;
+        *cold_slot(C, sh) = node;
       }
-      node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
-      sh -= 2u;
-      if (sh != 0u) *cold_slot(C, sh) = node;
     }
   }
   if (cand) {
