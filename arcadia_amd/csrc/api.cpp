@@ -34,8 +34,8 @@ struct arc_ctx {
   arc_brick64* d_bricks = nullptr;
   uint64_t n_nodes = 0, node_cap = 0, n_bricks = 0, brick_cap = 0;
   bool has_world = false;
-  // persistent-queue counters: words [0, kQueueWords) for the traversal kernel, [16, 16 +
-  // kQueueWords) for the path kernel, [32, 32 + kQueueWords) for the shadow walkers
+  // persistent queues (kQueueRegionWords words each): the traversal kernel's, the path kernel's,
+  // then the shadow walkers'
   uint32_t* d_queue = nullptr;
   // scratch for the host-pointer entry point
   arc_ray* d_rays = nullptr;
@@ -85,16 +85,17 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail((ctx), ARC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-static_assert(32u + arcdev::kQueueWords <= 64u, "queue words");
+constexpr size_t kQueueBytes = 3u * arcdev::kQueueRegionWords * sizeof(uint32_t);
 
-// Traversal kernel, measured on MI355X for primary and diffuse rays (full-frame world, ms per
-// primary + bounce step): 4 workgroups of 256 per CU 1.24, 5: 1.25, 6: 1.27, 3: 1.33, 2: 1.60,
-// 1: 2.68; refill at 16 idle lanes.
+// Traversal kernel, measured on MI355X with the 8-counter queue (full-frame world, ms per
+// primary / bounce launch, refill at 16 idle lanes): 6 workgroups of 256 per CU and windows of 64
+// rays 0.530 / 0.501; 4 workgroups 0.551 / 0.530; windows of 32 / 128 rays at 6 workgroups
+// 0.521 / 0.524 and 0.603 / 0.519.
 // blocks_per_cu 0 = as many workgroups as the kernel's occupancy keeps resident (the path kernel's
 // default: its register budget allows fewer).
 arc_launch_config default_config() {
   arc_launch_config c;
-  c.blocks_per_cu = 4;
+  c.blocks_per_cu = 6;
   c.block_threads = 256;
   c.refill_lanes = 16;
   c.batch_rays = 64;
@@ -133,11 +134,11 @@ int arc_create(int device, arc_ctx** out) {
   }
   c->n_cu = (uint32_t)prop.multiProcessorCount;
   c->lds_per_block = (uint32_t)prop.sharedMemPerBlock;
-  if (hipMalloc(&c->d_queue, 64 * sizeof(uint32_t)) != hipSuccess) {
+  if (hipMalloc(&c->d_queue, kQueueBytes) != hipSuccess) {
     delete c;
     return ARC_ERR_HIP;
   }
-  if (hipMemset(c->d_queue, 0, 64 * sizeof(uint32_t)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+  if (hipMemset(c->d_queue, 0, kQueueBytes) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
     (void)hipFree(c->d_queue);
     delete c;
     return ARC_ERR_HIP;
@@ -580,7 +581,7 @@ int arc_trace_shadow_device(arc_ctx* c, const arc_shadow_params* sp, const arc_r
   a.vis = d_vis;
   a.rec = d_rec;
   a.n = n;
-  a.queue = c->d_queue + 32;  // the walkers' own counters
+  a.queue = c->d_queue + 2u * arcdev::kQueueRegionWords;  // the walkers' own queue
   const arc_launch_config& cfg = c->cfg[0];
   uint32_t bpc = cfg.blocks_per_cu;
   if (bpc == 0) {
@@ -829,7 +830,7 @@ int batch_launch(arc_batch* b, arc_hit* d_hits, hipStream_t s) {
   a.bounce_max_dist_m = b->bounce_tmax;
   a.seed = b->seed;
   a.out = d_hits;
-  a.queue = c->d_queue + 16;  // the path kernel's own counters
+  a.queue = c->d_queue + arcdev::kQueueRegionWords;  // the path kernel's own queue
   uint32_t bpc = b->cfg.blocks_per_cu;
   if (bpc == 0) {
     hipError_t oe = arcdev::path_occupancy(b->cfg.block_threads, &bpc);

@@ -129,8 +129,8 @@ __device__ __forceinline__ void shadow_finish(float* __restrict__ vis, arc_shado
   }
 }
 
-// Persistent walker kernel: the ray queue and refill of k_trace (queue[0]: rays claimed,
-// queue[1]: finished workgroups, both zero between launches).
+// Persistent walker kernel: the ray queue and refill of k_trace (`queue`: the walkers' own queue
+// region, zero between launches).
 template <uint32_t MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_shadow(
     DevWorld w, DevParams p, const arc_ray* __restrict__ rays, float* __restrict__ vis, arc_shadow_rec* __restrict__ rec,
@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
   L.mat = 0u;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   uint32_t wb = 0u, we = 0u;  // wave-uniform: window of rays being handed out
+  uint32_t qc = home_counter(), qtried = 0u;
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
@@ -163,17 +164,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
       uint32_t assigned = 0u;
       uint32_t my = 0xFFFFFFFFu;
       while (assigned < nidle) {
-        if (wb >= we) {
-          uint32_t got = 0u;
-          if (lane == 0) got = atomicAdd(queue, batch);
-          got = rfl(__shfl(got, 0));
-          const uint32_t lo = umin_(got, n);
-          if (lo >= n) {
-            drained = true;
-            break;
-          }
-          wb = lo;
-          we = umin_(lo + batch, n);
+        if (wb >= we && !claim_window(queue, qc, qtried, batch, n, wb, we)) {
+          drained = true;
+          break;
         }
         const uint32_t take = umin_(nidle - assigned, we - wb);
         if (idle_lane && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);
@@ -191,16 +184,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (!idle_lane) shadow_iteration<MODE>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) shadow_finish<MODE>(vis, rec, p, L);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t done = atomicAdd(queue + 1, 1u);
-    if (done == gridDim.x - 1u) {
-      atomicExch(queue, 0u);
-      atomicExch(queue + 1, 0u);
-      __threadfence();
-    }
-  }
+  queue_finish(queue, threadIdx.x == 0u);
 }
 
 hipError_t shadow_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {

@@ -5,8 +5,9 @@
 // A persistent ray queue feeds one ray per lane. Each trip of a wave through its loop runs one
 // outer iteration of trace_world_64tree_from (traversal64.wgsl:302-553) for every live lane:
 // choose the restart node, descend at most four levels, then either record a hit or take one
-// exact DDA box step. The traversal is VALU-issue bound (pointer chasing with a long
-// per-step float sequence), so the kernel is shaped to cut the instructions a wave issues:
+// exact DDA box step. A wave's iteration is a chain of dependent node loads and a long exact float
+// sequence that the wave issues one instruction at a time (vector, scalar and branch alike), so
+// the kernel is shaped to cut the instructions a wave issues:
 //   * finished lanes are not resolved in place: the hit record (cell entry planes, tie-break
 //     normal, hash, AO) is computed once per refill for all finished lanes together, not in
 //     every wave trip in which some lane of 64 hits;
@@ -19,13 +20,12 @@
 // output: the cached-ancestor restart is equivalent to a root descent (the cached node at shift
 // s covers the current voxel's bits >= s+2 whenever the reference uses it).
 //
-// Queue: a wave claims windows of `batch` consecutive rays with one atomic on a single counter
-// and refills its idle lanes from the window, so all waves of all XCDs sweep the ray array
-// together and the rays in flight form one coherent band of screen tiles. Measured on MI355X:
-// per-XCD counters (the array cut into stripes of 1..4096 windows dealt round-robin to the XCDs,
-// each XCD draining its own stripes first) ran 15-25% slower at every stripe size, and 8
-// contiguous per-XCD segments 65% slower. The last workgroup to finish resets the counters,
-// which keeps a launch self-contained (no memset node: graph-safe).
+// Queue: a wave claims windows of `batch` consecutive rays with one atomic and refills its idle
+// lanes from the window. The claims go to kQueueCounters counters that deal out interleaved
+// windows (claim_window), so all waves of all XCDs still sweep the ray array together and the
+// rays in flight form one coherent band of screen tiles. Measured on MI355X: 8 contiguous per-XCD
+// segments ran 65% slower than one shared sweep. The last workgroup to finish resets the
+// counters, which keeps a launch self-contained (no memset node: graph-safe).
 #include <hip/hip_runtime.h>
 
 #include "raymath.h"
@@ -269,21 +269,8 @@ __device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const ar
   else store_compact<SEM>(reinterpret_cast<arc_hit*>(out), rays, L);
 }
 
-// The last workgroup out resets the counters for the next launch on this stream.
-__device__ __forceinline__ void queue_finish(uint32_t* queue, bool leader) {
-  __syncthreads();
-  if (leader) {
-    __threadfence();
-    const uint32_t done = atomicAdd(queue + 1, 1u);
-    if (done == gridDim.x - 1u) {
-      for (uint32_t s = 0; s < kQueueWords; ++s) atomicExch(queue + s, 0u);
-      __threadfence();
-    }
-  }
-}
-
 // Persistent ray-queue kernel.
-// queue[0]: rays claimed; queue[1]: finished workgroups (both zero between launches).
+// `queue`: one queue region (claim counters and finished workgroups, zero between launches).
 // Occupancy target: the compact variants fit 64 VGPRs (8 waves/SIMD) without spilling; the
 // full-record variants carry more live values through finalize and keep the default budget.
 template <uint32_t SEM, bool FULL, bool WATER>
@@ -302,6 +289,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
   L.idx = 0u;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   uint32_t wb = 0u, we = 0u;  // wave-uniform: window of rays being handed out
+  uint32_t qc = home_counter(), qtried = 0u;
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
@@ -319,17 +307,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
       uint32_t assigned = 0u;
       uint32_t my = 0xFFFFFFFFu;
       while (assigned < nidle) {
-        if (wb >= we) {
-          uint32_t got = 0u;
-          if (lane == 0) got = atomicAdd(queue, batch);
-          got = rfl(__shfl(got, 0));
-          const uint32_t lo = umin_(got, n);
-          if (lo >= n) {
-            drained = true;
-            break;
-          }
-          wb = lo;
-          we = umin_(lo + batch, n);
+        if (wb >= we && !claim_window(queue, qc, qtried, batch, n, wb, we)) {
+          drained = true;
+          break;
         }
         const uint32_t take = umin_(nidle - assigned, we - wb);
         if (idle_lane && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);
@@ -457,7 +437,7 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
   }
 }
 
-// queue[0]: slots claimed; queue[1]: finished workgroups (both zero between launches).
+// `queue`: one queue region (claim counters and finished workgroups, zero between launches).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
     DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
     uint32_t refill, uint32_t batch) {
@@ -473,6 +453,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   float tmax = 0.0f;
   uint32_t wb = 0u, we = 0u;
+  uint32_t qc = home_counter(), qtried = 0u;
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
@@ -487,16 +468,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         const uint32_t rank = (uint32_t)__popcll(nm & lt_mask);
         uint32_t assigned = 0u, my = 0xFFFFFFFFu;
         while (assigned < nneed) {
-          if (wb >= we) {
-            uint32_t got = 0u;
-            if (lane == 0) got = atomicAdd(queue, batch);
-            got = rfl(__shfl(got, 0));
-            if (got >= pc.n_items) {
-              drained = true;
-              break;
-            }
-            wb = got;
-            we = umin_(got + batch, pc.n_items);
+          if (wb >= we && !claim_window(queue, qc, qtried, batch, pc.n_items, wb, we)) {
+            drained = true;
+            break;
           }
           const uint32_t take = umin_(nneed - assigned, we - wb);
           if (needs && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);

@@ -8,8 +8,17 @@
 
 namespace arcdev {
 
-// Queue counters of one persistent launch: items claimed and finished workgroups.
-constexpr uint32_t kQueueWords = 2u;
+// Queue of one persistent launch: kQueueCounters claim counters kQueueStride words apart, each on
+// its own cache lines, and a finished-workgroup counter beside counter 0. Counter c hands out
+// windows c, c + K, c + 2K, ... of the item array (K = kQueueCounters), so the waves homed on the
+// K counters still sweep the array together while their claims go to K addresses: one atomic
+// address serialises its claims (about 12 ns each on MI355X), which bounded the launches of short
+// rays. All words are zero between launches.
+constexpr uint32_t kQueueCounters = 8u;
+constexpr uint32_t kQueueStride = 1024u;  // 4 KiB
+constexpr uint32_t kQueueDone = 1u;
+constexpr uint32_t kQueueRegionWords = kQueueCounters * kQueueStride;
+static_assert((kQueueCounters & (kQueueCounters - 1u)) == 0u, "kQueueCounters is a power of two");
 
 struct LaunchArgs {
   uint32_t grid[3];
@@ -21,7 +30,7 @@ struct LaunchArgs {
   void* out;
   uint64_t n;
   bool full;
-  uint32_t* queue;  // kQueueWords u32 counters, zero between launches
+  uint32_t* queue;  // kQueueRegionWords u32 words, zero between launches
   uint32_t blocks;
   uint32_t block_threads;
   uint32_t refill;
@@ -49,7 +58,7 @@ struct PathLaunchArgs {
   float bounce_max_dist_m;
   uint64_t seed;
   arc_hit* out;      // (1 + n_bounces) * n_items records
-  uint32_t* queue;   // kQueueWords u32 counters, zero between launches
+  uint32_t* queue;   // kQueueRegionWords u32 words, zero between launches
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream);
@@ -65,7 +74,7 @@ struct ShadowLaunchArgs {
   float* vis;
   arc_shadow_rec* rec;  // may be null
   uint64_t n;
-  uint32_t* queue;  // kQueueWords u32 counters, zero between launches
+  uint32_t* queue;  // kQueueRegionWords u32 words, zero between launches
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_shadow(const ShadowLaunchArgs& a, hipStream_t stream);

@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "trace_device.h"
+#include "trace_launch.h"
 
 namespace arcdev {
 
@@ -51,6 +52,46 @@ constexpr uint32_t kMaxBlock = 256u;
 constexpr uint32_t kColdSlots = 4u;
 
 __device__ __forceinline__ uint32_t rfl(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Persistent-queue claims (the queue layout is described with kQueueCounters). A wave starts on
+// counter home_counter() and claims one window of `batch` items per atomic; a counter whose next
+// window lies past the end stays exhausted, so the wave moves on to the next counter and is
+// drained after kQueueCounters consecutive exhausted counters. `c` and `tried` are wave-uniform.
+__device__ __forceinline__ uint32_t home_counter() {
+  return (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (kQueueCounters - 1u);
+}
+
+__device__ __forceinline__ bool claim_window(uint32_t* queue, uint32_t& c, uint32_t& tried, uint32_t batch,
+                                             uint32_t n, uint32_t& wb, uint32_t& we) {
+  for (;;) {
+    uint32_t got = 0u;
+    if (__lane_id() == 0u) got = atomicAdd(queue + c * kQueueStride, 1u);
+    got = rfl(__shfl(got, 0));
+    const uint64_t start = ((uint64_t)got * kQueueCounters + c) * batch;
+    if (start < n) {
+      wb = (uint32_t)start;
+      we = (uint32_t)(start + batch < n ? start + batch : n);
+      return true;
+    }
+    if (++tried == kQueueCounters) return false;
+    c = (c + 1u) & (kQueueCounters - 1u);
+  }
+}
+
+// The last workgroup out resets the queue for the next launch on this stream (no memset node:
+// graph-safe). `leader` is one thread of the workgroup.
+__device__ __forceinline__ void queue_finish(uint32_t* queue, bool leader) {
+  __syncthreads();
+  if (leader) {
+    __threadfence();
+    const uint32_t done = atomicAdd(queue + kQueueDone, 1u);
+    if (done == gridDim.x - 1u) {
+      for (uint32_t c = 0; c < kQueueCounters; ++c) atomicExch(queue + c * kQueueStride, 0u);
+      atomicExch(queue + kQueueDone, 0u);
+      __threadfence();
+    }
+  }
+}
 
 // float -> u32 for a value already clamped to x >= 0 (never NaN), whose result is then clamped
 // into a box below 2^32: min(x, 4294967040) (the largest float below 2^32) keeps the conversion
