@@ -132,7 +132,7 @@ __device__ __forceinline__ void shadow_finish(float* __restrict__ vis, arc_shado
 // Persistent walker kernel: the ray queue and refill of k_trace (`queue`: the walkers' own queue
 // region, zero between launches).
 template <uint32_t MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_shadow(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_shadow(
     DevWorld w, DevParams p, const arc_ray* __restrict__ rays, float* __restrict__ vis, arc_shadow_rec* __restrict__ rec,
     uint32_t n, uint32_t* __restrict__ queue, uint32_t refill, uint32_t batch) {
   __shared__ uint4 s_cold[kColdSlots * kMaxBlock];

@@ -151,8 +151,10 @@ __device__ __forceinline__ HitInfo resolve_hit(const Lane& L, float rdx, float r
   return h;
 }
 
+// `miss_t`: t_max * 16 of the lane's ray (world semantics), kept in LDS from the refill so a miss
+// record needs no second global read of the ray (a register would cost 15 VGPRs here).
 template <uint32_t SEM>
-__device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays, const Lane& L) {
+__device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays, const Lane& L, float miss_t) {
   uint4 v;
   const uint32_t steps = L.step > 0xFFFFu ? 0xFFFFu : L.step;
   if (fget(L.st, F_HIT, 1)) {
@@ -170,9 +172,7 @@ __device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays,
     v.z = (L.vz & 0xFFFFu) | ((L.mat & 0xFFu) << 16) | (face << 24);
     v.w = steps | ((ARC_HIT_FLAG_HIT | (fget(L.st, F_SHIFT, 3) > 0u ? ARC_HIT_FLAG_LOD : 0u)) << 16);
   } else {
-    float miss_t = 0.0f;
-    if (SEM == ARC_SEM_WORLD) miss_t = rays[L.idx].t_max * 16.0f;
-    v.x = __float_as_uint(miss_t);
+    v.x = __float_as_uint(SEM == ARC_SEM_WORLD ? miss_t : 0.0f);
     v.y = 0u;
     v.z = (uint32_t)ARC_FACE_NONE << 24;
     v.w = steps;
@@ -264,9 +264,9 @@ struct OutPtr<true> {
 
 template <uint32_t SEM, bool FULL>
 __device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const arc_ray* rays, const DevParams& p,
-                                         const Lane& L, const uint4* C) {
+                                         const Lane& L, const uint4* C, float miss_t) {
   if (FULL) store_full<SEM>(reinterpret_cast<arc_trace_result*>(out), rays, p, L, C);
-  else store_compact<SEM>(reinterpret_cast<arc_hit*>(out), rays, L);
+  else store_compact<SEM>(reinterpret_cast<arc_hit*>(out), rays, L, miss_t);
 }
 
 // Persistent ray-queue kernel.
@@ -274,7 +274,7 @@ __device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const ar
 // Occupancy target: the compact variants fit 64 VGPRs (8 waves/SIMD) without spilling; the
 // full-record variants carry more live values through finalize and keep the default budget.
 template <uint32_t SEM, bool FULL, bool WATER>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 8, 8))) void k_trace(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 6, 8))) void k_trace(
     DevWorld w, DevParams p, const arc_ray* __restrict__ rays,
                                                typename OutPtr<FULL>::T* __restrict__ out, uint32_t n,
                                                uint32_t* __restrict__ queue, uint32_t refill, uint32_t batch) {
@@ -290,6 +290,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
   uint32_t wb = 0u, we = 0u;  // wave-uniform: window of rays being handed out
   uint32_t qc = home_counter(), qtried = 0u;
+  __shared__ float s_miss[kMaxBlock];  // t_max * 16 of each lane's ray (compact world records)
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
     } else if (nidle >= refill) {
       // finalize finished lanes together, then refill every idle lane
       if (idle_lane && fget(L.st, F_HASRAY, 1)) {
-        finalize<SEM, FULL>(out, rays, p, L, C);
+        finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
         L.st = fput(L.st, F_HASRAY, 1, 0u);
       }
       const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
@@ -319,13 +320,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
       if (my != 0xFFFFFFFFu) {
         L.idx = my;
         const arc_ray r = rays[my];
+        if (!FULL && SEM == ARC_SEM_WORLD) s_miss[threadIdx.x] = r.t_max * 16.0f;
         ray_init<SEM>(w, p, r, L, C);
       }
       continue;  // re-ballot: freshly initialised lanes may already be done
     }
     if (!idle_lane) iteration<SEM, WATER>(w, p, L, C, s_roots, lds_roots);
   }
-  if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C);
+  if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
 

@@ -66,7 +66,7 @@ __device__ __forceinline__ bool claim_window(uint32_t* queue, uint32_t& c, uint3
   for (;;) {
     uint32_t got = 0u;
     if (__lane_id() == 0u) got = atomicAdd(queue + c * kQueueStride, 1u);
-    got = rfl(__shfl(got, 0));
+    got = __builtin_amdgcn_readlane(got, 0);  // lane 0's claim, into a scalar register
     const uint64_t start = ((uint64_t)got * kQueueCounters + c) * batch;
     if (start < n) {
       wb = (uint32_t)start;
