@@ -179,9 +179,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         const arc_ray r = rays[my];
         ray_init<ARC_SEM_WORLD, true>(w, p, r, L, C);  // sets mat = kEntered past the entry test
       }
-      continue;  // re-ballot: freshly initialised lanes may already be done
     }
-    if (!idle_lane) shadow_iteration<MODE>(w, p, L, C, s_roots, lds_roots);
+    // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
+    // `continue` back to the ballot made a second back-edge, and the register allocator then
+    // copied the whole lane state on every wave trip.)
+    if (fget(L.st, F_LIVE, 1)) shadow_iteration<MODE>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) shadow_finish<MODE>(vis, rec, p, L);
   queue_finish(queue, threadIdx.x == 0u);

@@ -323,9 +323,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
         if (!FULL && SEM == ARC_SEM_WORLD) s_miss[threadIdx.x] = r.t_max * 16.0f;
         ray_init<SEM>(w, p, r, L, C);
       }
-      continue;  // re-ballot: freshly initialised lanes may already be done
     }
-    if (!idle_lane) iteration<SEM, WATER>(w, p, L, C, s_roots, lds_roots);
+    // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
+    // `continue` back to the ballot made a second back-edge, and the register allocator then
+    // copied the whole lane state on every wave trip.)
+    if (fget(L.st, F_LIVE, 1)) iteration<SEM, WATER>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
@@ -481,10 +483,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         }
         if (my != 0xFFFFFFFFu) path_start(w, p0, pc, my, out, L, tmax, C);
       }
-      continue;
+    } else if (drained && nidle == 64u) {
+      break;  // drained, nothing pending, every lane idle
     }
-    if (drained && nidle == 64u) break;
-    if (!idle_lane)
+    // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
+    // straight on; the exit test above runs only when nothing was pending.
+    if (fget(L.st, F_LIVE, 1))
       iteration<ARC_SEM_WORLD, true>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots, lds_roots);
   }
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
