@@ -99,6 +99,14 @@ __device__ __forceinline__ void queue_finish(uint32_t* queue, bool leader) {
 // arithmetic (v_min_f32 + v_cvt_u32_f32), so the compiler may speculate it into selects.
 __device__ __forceinline__ uint32_t cvt_u32(float x) { return (uint32_t)fminf(x, 4294967040.0f); }
 
+// min(max(x, lo), hi) for lo <= hi in one instruction (the compiler forms v_med3_u32 only for
+// constant bounds). An ALU instruction, no memory access.
+__device__ __forceinline__ uint32_t umed3(uint32_t x, uint32_t lo, uint32_t hi) {
+  uint32_t r;
+  asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
 // t at which the ray crosses coordinate `c` on one axis: f32(c) * inv + (-o * inv).
 __device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
   const float q = (float)c * inv;
@@ -217,9 +225,9 @@ __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
   const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
   const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
   const uint32_t last = span - 1u;
-  const uint32_t cx = umin_(umax_(cvt_u32(ipx), bx), bx + last);
-  const uint32_t cy = umin_(umax_(cvt_u32(ipy), by), by + last);
-  const uint32_t cz = umin_(umax_(cvt_u32(ipz), bz), bz + last);
+  const uint32_t cx = umed3(cvt_u32(ipx), bx, bx + last);  // bx <= bx + last: boxes stay below 2^32
+  const uint32_t cy = umed3(cvt_u32(ipy), by, by + last);
+  const uint32_t cz = umed3(cvt_u32(ipz), bz, bz + last);
   const uint32_t fx = px ? ex : bx - 1u, fy = py ? ey : by - 1u, fz = pz ? ez : bz - 1u;
   const uint32_t nvx = is_x ? fx : cx;
   const uint32_t nvy = is_y ? fy : cy;
