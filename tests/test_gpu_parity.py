@@ -154,6 +154,21 @@ def test_launch_shapes_do_not_change_results(ctx, cfg):
         ctx.set_launch_config()
 
 
+@pytest.mark.parametrize("n,batch", [(1, 64), (7, 1), (63, 3), (8 * 64 - 1, 64), (8 * 64 + 1, 64),
+                                     (5000, 1 << 20), (5000, 7)])
+def test_queue_hands_out_every_ray_once(ctx, n, batch):
+    """Every ray of a launch gets its record whatever the window size against the eight claim
+    counters: fewer rays than one window, windows of one ray, fewer windows than counters (counters
+    empty from the start), sizes just off a multiple of windows x counters, one window for all."""
+    pools = W.caves_world(1)
+    rays = W.random_rays(n, (2, 2, 2), seed=100 + n, inside=False)
+    ctx.set_launch_config(blocks_per_cu=2, block_threads=256, refill_lanes=16, batch_rays=batch)
+    try:
+        _check(ctx, pools, A.TraceParams.exact(), rays, f"queue n={n} batch={batch}")
+    finally:
+        ctx.set_launch_config()
+
+
 def test_autotuner_keeps_results_bit_exact(ctx):
     torch = pytest.importorskip("torch")
     pools = W.caves_world(1)
