@@ -9,6 +9,9 @@
  *
  * Ownership: the caller owns host arrays; the context owns the device pools.
  * A context is not thread-safe: use one per host thread or lock around it.
+ * Streams: each context keeps one self-resetting work queue per kernel family (traversal, path
+ * batch, shadow walkers) in device memory, so two launches of one family on one context must
+ * not run at the same time on different streams (order them, or use a context per stream).
  */
 #ifndef ARCADIA_HIP_H
 #define ARCADIA_HIP_H
