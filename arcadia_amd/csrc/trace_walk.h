@@ -32,6 +32,7 @@ struct Lane {
   float dx, dy, dz;  // direction used for inside points (WORLD: dir, CHUNK: safe_dir)
   float ix, iy, iz;  // 1 / safe_dir
   float nx, ny, nz;  // -origin * inv
+  uint32_t mx, my, mz;  // 0xFFFFFFFF where inv >= 0 (the box step's exit planes), else 0
   float t, tl;       // t, t_limit
   uint32_t vx, vy, vz;
   uint32_t diff;     // prev_vi ^ vi of the coming iteration
@@ -215,8 +216,7 @@ __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4*
 __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
   const uint32_t mk = 0u - span;
   const uint32_t bx = L.vx & mk, by = L.vy & mk, bz = L.vz & mk;
-  const bool px = L.ix >= 0.0f, py = L.iy >= 0.0f, pz = L.iz >= 0.0f;
-  const uint32_t ex = px ? bx + span : bx, ey = py ? by + span : by, ez = pz ? bz + span : bz;
+  const uint32_t ex = bx + (span & L.mx), ey = by + (span & L.my), ez = bz + (span & L.mz);
   const float tx = plane_t(ex, L.ix, L.nx), ty = plane_t(ey, L.iy, L.ny), tz = plane_t(ez, L.iz, L.nz);
   const float t_exit = fminf(fminf(tx, ty), tz);
   const float next_t = fmaxf(L.t, t_exit);
@@ -228,7 +228,8 @@ __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
   const uint32_t cx = umed3(cvt_u32(ipx), bx, bx + last);  // bx <= bx + last: boxes stay below 2^32
   const uint32_t cy = umed3(cvt_u32(ipy), by, by + last);
   const uint32_t cz = umed3(cvt_u32(ipz), bz, bz + last);
-  const uint32_t fx = px ? ex : bx - 1u, fy = py ? ey : by - 1u, fz = pz ? ez : bz - 1u;
+  const uint32_t fx = (ex & L.mx) | ((bx - 1u) & ~L.mx), fy = (ey & L.my) | ((by - 1u) & ~L.my),
+                 fz = (ez & L.mz) | ((bz - 1u) & ~L.mz);
   const uint32_t nvx = is_x ? fx : cx;
   const uint32_t nvy = is_y ? fy : cy;
   const uint32_t nvz = (is_x || is_y) ? cz : fz;
@@ -330,6 +331,9 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
   L.nx = (-L.ox) * L.ix;
   L.ny = (-L.oy) * L.iy;
   L.nz = (-L.oz) * L.iz;
+  L.mx = L.ix >= 0.0f ? 0xFFFFFFFFu : 0u;
+  L.my = L.iy >= 0.0f ? 0xFFFFFFFFu : 0u;
+  L.mz = L.iz >= 0.0f ? 0xFFFFFFFFu : 0u;
   L.diff = 0xFFFFFFFFu;  // prev_vi starts at u32::MAX
   if (iteration_guards_fail<SEM>(w, p, L)) return;
   L.st = fput(fput(L.st, F_CSHIFT, 3, 6u), F_LIVE, 1, 1u);
