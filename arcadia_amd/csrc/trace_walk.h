@@ -115,17 +115,14 @@ __device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
 }
 
 // LOD min_shift (traversal64.wgsl:355-370); `sidx` is the iteration index.
-// The three classes as one branch-free test: class 1 reaches shift 4 past lod_t_far or
-// lod_step and shift 2 past lod_t only; classes 0 and 2 reach shift 2 past lod_t or lod_step and
-// never 4 (t > +inf and sidx > 0xFFFFFFFF are false, as is every comparison with a NaN t).
+// lod_bias is a launch parameter, so the class test is a scalar branch: class 1 reaches shift 4
+// past lod_t_far or lod_step and shift 2 past lod_t only; classes 0 and 2 reach shift 2 past lod_t
+// or lod_step and never 4. Two vector compares per step for classes 0 and 2 instead of four.
 template <uint32_t SEM>
 __device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, uint32_t sidx) {
   if (SEM == ARC_SEM_CHUNK) return 0u;
-  const bool c1 = p.lod_bias == 1u;
-  const float t4 = c1 ? p.lod_t_far : __builtin_huge_valf();
-  const uint32_t s4 = c1 ? p.lod_step : 0xFFFFFFFFu;
-  const uint32_t s2 = c1 ? 0xFFFFFFFFu : p.lod_step;
-  return (t > t4 || sidx > s4) ? 4u : ((t > p.lod_t || sidx > s2) ? 2u : 0u);
+  if (p.lod_bias == 1u) return (t > p.lod_t_far || sidx > p.lod_step) ? 4u : (t > p.lod_t ? 2u : 0u);
+  return (t > p.lod_t || sidx > p.lod_step) ? 2u : 0u;
 }
 
 // Guards at the top of an outer iteration (traversal64.wgsl:302-310 / raycast_cpu.rs:78-84).
