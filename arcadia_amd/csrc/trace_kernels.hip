@@ -84,6 +84,7 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       }
     }
   }
+  bool hit = false;
   if (cand) {  // :441-538
     const uint32_t b = child_bit(L, sh);
     uint32_t mat = node.w >> 24;
@@ -92,17 +93,19 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       mat = __builtin_amdgcn_ubfe(word, (b & 3u) * 8u, 8);
     }
     const bool skip_water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && mat == ARC_MAT_WATER;
-    if (!skip_water && mat != ARC_MAT_AIR) {  // hit: resolved at finalize (:467-523)
-      L.mat = mat;
-      L.node = node;
-      L.st = fput(fput(fput(fput(st, F_CSHIFT, 3, sh), F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
-      return;
-    }
+    hit = !skip_water && mat != ARC_MAT_AIR;  // hit: resolved at finalize (:467-523)
+    if (hit) L.mat = mat;
   }
+  // One way out of the iteration (a hit returns no earlier), so the lane state has one
+  // definition per field where the iteration ends.
   L.node = node;
-  box_step(L, span_mul << sh);
   st = fput(st, F_CSHIFT, 3, sh);
-  if (iteration_guards_fail<SEM>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
+  if (hit) {
+    st = fput(fput(fput(st, F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
+  } else {
+    box_step(L, span_mul << sh);
+    if (iteration_guards_fail<SEM>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
+  }
   L.st = st;
 }
 
