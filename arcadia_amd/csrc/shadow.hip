@@ -67,6 +67,7 @@ __device__ __forceinline__ void shadow_iteration(const DevWorld& w, const DevPar
       }
     }
   }
+  bool hit = false;
   if (cand) {
     uint32_t mat = node.w >> 24;
     if (MODE == ARC_SHADOW_DI && sh == 0u && (node.z & (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG)) == ARC_IS_LEAF_FLAG) {
@@ -74,18 +75,21 @@ __device__ __forceinline__ void shadow_iteration(const DevWorld& w, const DevPar
       const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
       mat = __builtin_amdgcn_ubfe(word, (b & 3u) * 8u, 8);
     }
-    if (mat != ARC_MAT_WATER) {  // water cells are stepped over (span 1 << sh)
+    hit = mat != ARC_MAT_WATER;  // water cells are stepped over (span 1 << sh)
+    if (hit) {
       const bool lit = MODE == ARC_SHADOW_DI && is_emissive_mat(mat) && L.t >= L.tl - 16.0f;
       L.mat = lit ? ARC_SHADOW_EMITTER : ARC_SHADOW_BLOCKED;
-      L.node = node;
-      L.st = fput(fput(fput(st, F_CSHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
-      return;
     }
   }
+  // one way out, as in the traversal's iteration
   L.node = node;
-  box_step(L, span_mul << sh);
   st = fput(st, F_CSHIFT, 3, sh);
-  if (iteration_guards_fail<ARC_SEM_WORLD>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
+  if (hit) {
+    st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 1u);
+  } else {
+    box_step(L, span_mul << sh);
+    if (iteration_guards_fail<ARC_SEM_WORLD>(w, p, L)) st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 0u);
+  }
   L.st = st;
 }
 
