@@ -79,14 +79,15 @@ def cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads: int, budget_
         O.trace_meters(ow, sample, threads)
         times.append(time.perf_counter() - t)
     cpu_rate = len(sample) / float(np.median(times))
-    # algorithmic bytes per ray from the WGSL-semantics counters (strided samples per class)
+    # algorithmic bytes per ray from the WGSL-semantics counters (strided samples per class); the
+    # same oracle records are kept to check the timed launches' records bit for bit
     def bytes_per_ray(rays, params):
-        s = rays[:: max(1, len(rays) // 200000)]
-        _, cnt = O.trace_world(ow, params, s, threads)
+        idx = np.arange(0, len(rays), max(1, len(rays) // 200000))
+        full, cnt = O.trace_world(ow, params, rays[idx], threads)
         b = 16.0 * cnt["node_loads"] + 4.0 * cnt["brick_loads"] + 4.0 * cnt["root_loads"] + 32.0 + 16.0
-        return float(b.mean()), float(cnt["steps"].mean())
-    bp, sp = bytes_per_ray(prays, p_primary)
-    bb, sb = bytes_per_ray(brays, p_bounce)
+        return float(b.mean()), float(cnt["steps"].mean()), (idx, O.compact(full))
+    bp, sp, ref_p = bytes_per_ray(prays, p_primary)
+    bb, sb, ref_b = bytes_per_ray(brays, p_bounce)
     return {
         "value": cpu_rate / 1e9,
         "unit": "Grays/s",
@@ -94,7 +95,7 @@ def cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads: int, budget_
         "kind": "port",
         "sample": f"{len(sample)} rays (every {stride}th of the {len(allr)} primary+bounce rays), "
                   f"WorldMap::raycast_meters restated in C, median of 3",
-    }, {"primary": bp, "bounce": bb, "steps_primary": sp, "steps_bounce": sb}
+    }, {"primary": bp, "bounce": bb, "steps_primary": sp, "steps_bounce": sb, "ref_primary": ref_p, "ref_bounce": ref_b}
 
 
 def batched_leg(args, threads, stream):
@@ -769,6 +770,16 @@ def main() -> int:
                            steps_per_ray=round(bytes_per["steps_bounce"], 2)),
         }
         out["cpu_baseline"] = cpu
+        # the records of the last timed launches against the oracle on the strided samples
+        got_p = d_phits.cpu().numpy().view(A.HIT_DTYPE)
+        got_b = d_bhits.cpu().numpy().view(A.HIT_DTYPE)[:n_b]
+        exact = {}
+        for kind, got in (("primary", got_p), ("bounce", got_b)):
+            idx, ref = bytes_per["ref_" + kind]
+            exact[kind] = bool(np.array_equal(got[idx].view(np.uint32), ref.view(np.uint32)))
+            exact[kind + "_rays"] = int(len(idx))
+        out["oracle_sample_exact"] = exact
+        print(f"[bench] timed records vs oracle: {exact}", file=sys.stderr, flush=True)
     if not args.no_shadow:
         out["shadow"] = shadow_leg(args, ctx, pools, prays, phits, stream, threads)
     if not args.no_rigid:
