@@ -87,10 +87,9 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
 
 constexpr size_t kQueueBytes = 3u * arcdev::kQueueRegionWords * sizeof(uint32_t);
 
-// Traversal kernel, measured on MI355X with the 8-counter queue (full-frame world, ms per
-// primary / bounce launch, refill at 16 idle lanes): 6 workgroups of 256 per CU and windows of 64
-// rays 0.530 / 0.501; 4 workgroups 0.551 / 0.530; windows of 32 / 128 rays at 6 workgroups
-// 0.521 / 0.524 and 0.603 / 0.519.
+// Traversal kernel, measured on MI355X (full-frame world, ms per primary / bounce launch, refill
+// at 16 idle lanes, windows of 64 rays): 6 workgroups of 256 per CU 0.450 / 0.429; 5: 0.464 /
+// 0.437; 7: 0.451 / 0.436. Windows of 32 at 6 workgroups: 0.452 / 0.459.
 // blocks_per_cu 0 = as many workgroups as the kernel's occupancy keeps resident (the path kernel's
 // default: its register budget allows fewer).
 arc_launch_config default_config() {
