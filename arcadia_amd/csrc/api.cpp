@@ -581,7 +581,9 @@ int arc_trace_shadow_device(arc_ctx* c, const arc_shadow_params* sp, const arc_r
   a.rec = d_rec;
   a.n = n;
   a.queue = c->d_queue + 2u * arcdev::kQueueRegionWords;  // the walkers' own queue
-  const arc_launch_config& cfg = c->cfg[0];
+  // The walkers keep the default shape: the per-class traversal shapes (and the autotuner, which
+  // sets them for long rays) measured slower on these short rays (sun 0.220 vs 0.199 ms).
+  const arc_launch_config cfg = default_config();
   uint32_t bpc = cfg.blocks_per_cu;
   if (bpc == 0) {
     hipError_t oe = arcdev::shadow_occupancy(cfg.block_threads, &bpc);

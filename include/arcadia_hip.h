@@ -243,7 +243,8 @@ typedef struct arc_shadow_rec {
   uint8_t reserved;
 } arc_shadow_rec;
 /* Device pointers, asynchronous on `stream`: d_vis[i] = visibility of ray i (0.0 / 1.0);
- * d_rec may be NULL. */
+ * d_rec may be NULL. The walkers launch with the default shape (6 workgroups of 256 per CU,
+ * refill 16, windows of 64 rays), whatever arc_set_launch_config / arc_autotune set for traces. */
 int arc_trace_shadow_device(arc_ctx* ctx, const arc_shadow_params* params, const arc_ray* d_rays, uint64_t n,
                             float* d_vis, arc_shadow_rec* d_rec, void* stream);
 
