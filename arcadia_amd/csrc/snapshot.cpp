@@ -184,7 +184,8 @@ int validate(Snapshot& s, std::string& why) {
     }
     prev_code = code;
     const uint64_t bytes = snap_payload_bytes(e);
-    if (e.offset % h.align != 0 || e.offset < prev_end || e.offset + bytes > s.size) {
+    // offset and size are tested apart: a forged offset near 2^64 must not wrap the sum
+    if (e.offset % h.align != 0 || e.offset < prev_end || e.offset > s.size || bytes > s.size - e.offset) {
       why = "index entry " + std::to_string(k) + ": payload offset out of place";
       return ARC_ERR_CORRUPT;
     }

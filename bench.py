@@ -43,6 +43,25 @@ def measured_traffic(kind: str):
     return None
 
 
+def measured_counters() -> dict:
+    """L2 hit rate, VALU busy and lanes active of the traversal kernels on this workload, from the
+    rocprofv3 counter summary in profiles/ (key: value lines); empty when the summary is absent."""
+    p = ROOT / "profiles" / "k_trace_fullframe_counters.txt"
+    out = {}
+    if not p.exists():
+        return out
+    for line in p.read_text().splitlines():
+        if line.startswith("#") or ":" not in line:
+            continue
+        k, v = (x.strip() for x in line.split(":", 1))
+        try:
+            out[k] = float(v)
+        except ValueError:
+            out[k] = v
+    out["source"] = "profiles/k_trace_fullframe_counters.txt"
+    return out
+
+
 def build_workload(seed: int, nthreads: int):
     import arcadia_amd as A
 
@@ -56,29 +75,54 @@ def build_workload(seed: int, nthreads: int):
     return world, pools, prays, time.time() - t0
 
 
-def cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads: int, budget_s: float):
-    """Reference CPU traversal (WorldMap::raycast_meters restated in C, the oracle) on a bounded,
-    evenly strided sample of the same rays; also the per-ray load counters that give the
-    algorithmic bytes of the traversal (trace_world_64tree_from restated in C)."""
+def host_cpu_info():
+    """Host cores the CPU baseline runs on: the affinity mask, capped by the box's declared CPU share
+    (OMP_NUM_THREADS, set on GPU boxes to the cores one GPU's jobs may use), plus nproc and the model."""
+    aff = len(os.sched_getaffinity(0))
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(share)) if share.isdigit() and int(share) > 0 else aff
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity_cores": aff, "omp_num_threads": share or None,
+                     "model": model}
+
+
+def time_cpu(fn, rays, budget_s: float):
+    """CPU baseline protocol: one full warm-up pass over the sample, then the median of 5 timed
+    passes. The sample is the whole ray set when 6 passes fit `budget_s`, else an even stride of it.
+    Returns (rays/s, stride, sample size)."""
+    probe = rays[:: max(1, len(rays) // 50000)]
+    t = time.perf_counter()
+    fn(probe)
+    rate = len(probe) / max(time.perf_counter() - t, 1e-6)
+    n = int(min(len(rays), max(len(probe), rate * budget_s / 6.0)))
+    stride = max(1, -(-len(rays) // n))
+    sample = np.ascontiguousarray(rays[::stride])
+    fn(sample)  # warm-up pass
+    times = []
+    for _ in range(5):
+        t = time.perf_counter()
+        fn(sample)
+        times.append(time.perf_counter() - t)
+    return len(sample) / float(np.median(times)), stride, len(sample)
+
+
+def cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads: int, cpu_info: dict, budget_s: float):
+    """Reference CPU traversal (WorldMap::raycast_meters restated in C, the oracle) on the same rays;
+    also the per-ray load counters that give the algorithmic bytes of the traversal
+    (trace_world_64tree_from restated in C)."""
     from oracle import oracle as O
 
     ow = O.World(*pools)
     allr = np.concatenate([prays, brays])
-    # probe speed on a small sample, then size the timed sample to the budget
-    probe = allr[:: max(1, len(allr) // 50000)]
-    t = time.perf_counter()
-    O.trace_meters(ow, probe, threads)
-    rate = len(probe) / max(time.perf_counter() - t, 1e-6)
-    n = int(min(len(allr), max(len(probe), rate * budget_s)))
-    stride = max(1, len(allr) // n)
-    sample = allr[::stride]
-    O.trace_meters(ow, sample[: min(len(sample), 20000)], threads)  # warm-up
-    times = []
-    for _ in range(3):
-        t = time.perf_counter()
-        O.trace_meters(ow, sample, threads)
-        times.append(time.perf_counter() - t)
-    cpu_rate = len(sample) / float(np.median(times))
+    cpu_rate, stride, n_sample = time_cpu(lambda r: O.trace_meters(ow, r, threads), allr, budget_s)
     # algorithmic bytes per ray from the WGSL-semantics counters (strided samples per class); the
     # same oracle records are kept to check the timed launches' records bit for bit
     def bytes_per_ray(rays, params):
@@ -93,9 +137,87 @@ def cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads: int, budget_
         "unit": "Grays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{len(sample)} rays (every {stride}th of the {len(allr)} primary+bounce rays), "
-                  f"WorldMap::raycast_meters restated in C, median of 3",
+        "sample": f"{n_sample} rays (every {stride}th of the {len(allr)} primary+bounce rays), "
+                  f"WorldMap::raycast_meters restated in C (-O3 -ffp-contract=off), one warm-up pass, median of 5",
+        "host": cpu_info,
     }, {"primary": bp, "bounce": bb, "steps_primary": sp, "steps_bounce": sb, "ref_primary": ref_p, "ref_bounce": ref_b}
+
+
+def c1_poses(seed: int):
+    """4 seeded poses above the 256^3 terrain: eye y in [13, 15.5] m, pitch -20..-60 deg."""
+    rng = np.random.default_rng(seed)
+    poses = []
+    for _ in range(4):
+        eye = (float(rng.uniform(2.0, 14.0)), float(rng.uniform(13.0, 15.5)), float(rng.uniform(2.0, 14.0)))
+        poses.append((eye, float(rng.uniform(0, 2 * np.pi)), float(np.deg2rad(rng.uniform(-60.0, -20.0)))))
+    return poses
+
+
+def c1_leg(args, stream, threads, cpu_info):
+    """Plain traversal, small: one 256^3 chunk (4-level tree) from a seeded fBm heightmap, 4 poses of
+    640x360 primary rays, raycast_chunk_tree semantics (512-step cap, voxel units) on the GPU, and the
+    same rays through the C restatement of raycast_chunk_tree on the host cores."""
+    import torch
+
+    import arcadia_amd as A
+    from oracle import oracle as O
+
+    w = A.HostWorld((1, 1, 1))
+    w.generate(A.gen_params(A.ARC_GEN_HEIGHTMAP, seed=args.seed), nthreads=threads)
+    w.pack()
+    pools = w.pools()
+    rays = []
+    for eye, yaw, pitch in c1_poses(args.seed):
+        cam = A.camera_look(eye, yaw, pitch, 90.0, 640, 360)
+        r, _ = A.primary_rays(cam, 640, 360, 250.0)
+        rays.append(r)
+    rays = np.concatenate(rays)
+    for k in ("ox", "oy", "oz"):
+        rays[k] *= 16.0
+    rays["t_max"] = 250.0 * 16.0
+    n = len(rays)
+    params = A.TraceParams.chunk()
+    ctx = A.Context(0)
+    ctx.upload(*pools)
+    sptr = stream.cuda_stream
+    d_rays = torch.from_numpy(rays.view(np.uint8)).cuda()
+    d_hits = torch.zeros(n * 16, dtype=torch.uint8, device="cuda")
+    if not args.no_autotune:
+        ctx.autotune(params, d_rays.data_ptr(), n, d_hits.data_ptr(), sptr)
+    for _ in range(3):
+        ctx.trace_device(params, d_rays.data_ptr(), n, d_hits.data_ptr(), sptr)
+    torch.cuda.synchronize()
+    k = max(5, args.steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(k):
+        ctx.trace_device(params, d_rays.data_ptr(), n, d_hits.data_ptr(), sptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / k
+    got = d_hits.cpu().numpy().view(A.HIT_DTYPE)
+    ow = O.World(*pools)
+    full, cnt = O.trace_chunk(ow, rays, threads)
+    exact = bool(np.array_equal(got.view(np.uint32), O.compact(full).view(np.uint32)))
+    out = {
+        "metric": "Grays/s raycast_chunk_tree (256^3 heightmap chunk, 4 poses 640x360, 512-step cap)",
+        "value": round(n / (ms * 1e-3) / 1e9, 4),
+        "unit": "Grays/s",
+        "ms": round(ms, 4),
+        "rays": n,
+        "hit_fraction": round(float((got["flags"] & 1).mean()), 4),
+        "steps_per_ray": round(float(cnt["steps"].mean()), 2),
+        "oracle_exact": exact,
+        "launch": ctx.launch_config(0),
+    }
+    if not args.no_cpu_baseline:
+        rate, stride, n_sample = time_cpu(lambda r: O.trace_chunk(ow, r, threads), rays, args.cpu_budget)
+        out["cpu_baseline"] = {
+            "value": rate / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
+            "sample": f"{n_sample} rays (every {stride}th of {n}), raycast_chunk_tree restated in C "
+                      f"(-O3 -ffp-contract=off), one warm-up pass, median of 5", "host": cpu_info}
+    ctx.close()
+    return out
 
 
 def batched_leg(args, threads, stream):
@@ -147,6 +269,8 @@ def batched_leg(args, threads, stream):
     t = hits.view(torch.float32)[0::4].reshape(3, b.slots)
     traced = (t != -16.0).sum(dim=1).cpu().numpy().astype(np.int64)
     n_rays = int(traced.sum())
+    exact = batched_sample_check(pools, views, hits, b.slots, p0, pb, args.seed, 2, 26.0, 20000, threads)
+    print(f"[bench] batched: sampled slots vs oracle {exact}", file=sys.stderr, flush=True)
     out = {
         "metric": "Grays/s batched views (primary + 2 diffuse bounces, one persistent launch)",
         "value": round(n_rays / (ms * 1e-3) / 1e9, 4),
@@ -166,10 +290,47 @@ def batched_leg(args, threads, stream):
         "launch": b.launch_config(),
         "autotuned": tuned is not None,
         "graph": "hipGraph of the single path launch, replayed per frame",
+        "oracle_exact": exact,
     }
     b.close()
     ctx.close()
     return out
+
+
+def batched_sample_check(pools, views, hits, n, p0, pb, seed, n_bounces, bounce_max_dist_m, count, threads):
+    """Every record of `count` random pixel slots of the last replay against the oracle: each slot's
+    primary ray is regenerated on the host (its view's padded 8x8-tile order), traced by the oracle,
+    and its bounce chain re-derived from the oracle's hit with the slot's own bounce index."""
+    import torch
+
+    import arcadia_amd as A
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(seed + 7)
+    sample = np.sort(rng.choice(n, min(n, count), replace=False))
+    rays = np.zeros(len(sample), A.RAY_DTYPE)
+    first = 0
+    for cam, w, h, md in views:
+        nv = ((w + 7) // 8) * ((h + 7) // 8) * 64
+        sel = (sample >= first) & (sample < first + nv)
+        if sel.any():
+            rays[sel] = A.view_rays(cam, w, h, md)[sample[sel] - first]
+        first += nv
+    idx = torch.from_numpy(np.concatenate([d * n + sample for d in range(n_bounces + 1)]).astype(np.int64)).cuda()
+    got = hits.view(torch.int32).reshape(-1, 4)[idx].cpu().numpy().view(A.HIT_DTYPE).reshape(-1)
+    ow = O.World(*pools)
+    cur, exact = rays, {}
+    for depth in range(n_bounces + 1):
+        full, _ = O.trace_world(ow, p0 if depth == 0 else pb, cur, threads)
+        ref = O.compact(full)
+        exact[f"depth{depth}"] = bool(np.array_equal(got[depth * len(sample):(depth + 1) * len(sample)].view(np.uint32),
+                                                     ref.view(np.uint32)))
+        if depth < n_bounces:
+            cur = np.concatenate([A.bounce_rays_dense(cur[k:k + 1], ref[k:k + 1], index_base=depth * n + int(s),
+                                                      seed=seed, max_dist_m=bounce_max_dist_m)
+                                  for k, s in enumerate(sample)])
+    exact["slots"] = int(len(sample))
+    return exact
 
 
 def shadow_leg(args, ctx, pools, prays, phits, stream, threads):
@@ -618,6 +779,7 @@ def main() -> int:
                     help="5 timed steps after 1 warm-up, no CPU leg and no other workloads (for rocprofv3); the "
                          "autotuner still runs, so the timed launches use the same shapes as a full run")
     ap.add_argument("--no-autotune", action="store_true", help="use the default launch shapes")
+    ap.add_argument("--no-c1", action="store_true", help="skip the single-chunk raycast_chunk_tree workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-walker workload")
     ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-body workload")
@@ -638,6 +800,7 @@ def main() -> int:
     if args.profile:
         args.steps, args.warmup, args.no_cpu_baseline = min(args.steps, 5), 1, True
         args.no_batched = True
+        args.no_c1 = True
         args.no_shadow = True
         args.no_rigid = True
         args.no_snapshot = True
@@ -649,7 +812,7 @@ def main() -> int:
 
     import arcadia_amd as A
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads, cpu_info = host_cpu_info()
     world, pools, prays, t_build = build_workload(args.seed, threads)
     print(f"[bench] world built in {t_build:.1f}s: {len(pools[2])} nodes, {len(pools[3])} bricks", file=sys.stderr,
           flush=True)
@@ -745,7 +908,8 @@ def main() -> int:
         },
     }
     if not args.no_cpu_baseline:
-        cpu, bytes_per = cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads, args.cpu_budget)
+        cpu, bytes_per = cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads, cpu_info,
+                                      args.cpu_budget)
         # Two instantiations of the traversal kernel run per step: k_trace<WORLD, compact, water off>
         # on the primary rays and k_trace<WORLD, compact, water on> on the bounce rays. The roofline
         # is the primary-ray kernel's; the bounce kernel's figures sit beside it.
@@ -755,19 +919,31 @@ def main() -> int:
                     "traffic": measured_traffic(kind), "bytes_per_ray": round(bytes_per[kind], 2),
                     "avg_launch_ms": round(ms, 4)}
         rp, rb = roof("primary", n_p, ms_p), roof("bounce", n_b, ms_b)
+        ev = measured_counters()
         out["roofline"] = {
-            "bound": "hbm",
+            # what bounds the kernel, from its rocprofv3 counters: VALU busy near saturation with HBM
+            # traffic a few percent of peak is issue-bound; `achieved` / `frac` stay the algorithmic
+            # (logical) bytes per launch over HBM peak
+            "bound": ev.get("bound", "unmeasured"),
             "achieved": rp["achieved"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": rp["frac"],
             "traffic": rp["traffic"],
+            "hbm_gbs": None if rp["traffic"] is None else round(rp["traffic"] / (ms_p * 1e-3) / 1e9, 1),
+            "l2_hit": ev.get("primary_l2_hit"),
+            "valu_busy": ev.get("primary_valu_busy"),
+            "lanes_active": ev.get("primary_lanes_active"),
             "kernel": "k_trace<WORLD, compact, ignore_water off> on the primary rays",
             "bytes_per_ray": rp["bytes_per_ray"],
             "avg_launch_ms": rp["avg_launch_ms"],
             "steps_per_ray": round(bytes_per["steps_primary"], 2),
             "bounce": dict(rb, kernel="k_trace<WORLD, compact, ignore_water on> on the bounce rays",
-                           steps_per_ray=round(bytes_per["steps_bounce"], 2)),
+                           steps_per_ray=round(bytes_per["steps_bounce"], 2),
+                           hbm_gbs=None if rb["traffic"] is None else round(rb["traffic"] / (ms_b * 1e-3) / 1e9, 1),
+                           l2_hit=ev.get("bounce_l2_hit"), valu_busy=ev.get("bounce_valu_busy"),
+                           lanes_active=ev.get("bounce_lanes_active")),
+            "evidence": ev.get("source"),
         }
         out["cpu_baseline"] = cpu
         # the records of the last timed launches against the oracle on the strided samples
@@ -780,6 +956,10 @@ def main() -> int:
             exact[kind + "_rays"] = int(len(idx))
         out["oracle_sample_exact"] = exact
         print(f"[bench] timed records vs oracle: {exact}", file=sys.stderr, flush=True)
+    if not args.no_c1:
+        out["c1"] = c1_leg(args, stream, threads, cpu_info)
+        print(f"[bench] c1: {out['c1']['value']} Grays/s ({out['c1']['ms']} ms), exact {out['c1']['oracle_exact']}",
+              file=sys.stderr, flush=True)
     if not args.no_shadow:
         out["shadow"] = shadow_leg(args, ctx, pools, prays, phits, stream, threads)
     if not args.no_rigid:
@@ -801,8 +981,39 @@ def main() -> int:
               f"ms nrc {c['nrc_ms_per_frame']} sharc {c['sharc_ms_per_frame']}", file=sys.stderr, flush=True)
     if not args.no_destruction:
         out["destruction"] = destruction_leg(args, threads)
+    failures = parity_failures(out)
+    if failures:
+        out["parity_failures"] = failures
     print(json.dumps(out), flush=True)
+    if failures:
+        print(f"[bench] records differ from the oracle: {failures}", file=sys.stderr, flush=True)
+        return 1
     return 0
+
+
+def parity_failures(out: dict) -> list:
+    """Every oracle comparison the run made that did not match (the run then exits non-zero)."""
+    bad = []
+    for k, v in out.get("oracle_sample_exact", {}).items():
+        if isinstance(v, bool) and not v:
+            bad.append(f"full-frame {k}")
+    if "c1" in out and not out["c1"]["oracle_exact"]:
+        bad.append("c1")
+    for walker in ("sun", "di"):
+        if "shadow" in out and not out["shadow"][walker]["oracle_sample_exact"]:
+            bad.append(f"shadow {walker}")
+    if "rigid" in out and not out["rigid"]["oracle_sample_exact"]:
+        bad.append("rigid")
+    if "snapshot" in out and not out["snapshot"]["stream_pools_equal"]:
+        bad.append("snapshot pools")
+    if "restir" in out and not out["restir"]["frame0_oracle_exact"]:
+        bad.append("restir frame 0")
+    if "destruction" in out and out["destruction"]["frames_bit_exact"] != out["destruction"]["frames"]:
+        bad.append("destruction")
+    for k, v in out.get("batched", {}).get("oracle_exact", {}).items():
+        if isinstance(v, bool) and not v:
+            bad.append(f"batched {k}")
+    return bad
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
 /*
  * CPU oracle for the 64-tree DDA traversal — TEST INFRASTRUCTURE ONLY (see arc_oracle.h).
  *
- * Build: gcc -O2 -ffp-contract=off -fno-fast-math (oracle/Makefile). Every float operation
+ * Build: gcc -O3 -ffp-contract=off -fno-fast-math (oracle/Makefile). Every float operation
  * below is one IEEE-754 binary32 operation in source order, with no contraction, which is
  * how Rust evaluates raycast_cpu.rs and how the WGSL source reads once FMA fusion is ruled
  * out. min/max return the non-NaN operand (fminf/fmaxf), float->u32 saturates with NaN->0

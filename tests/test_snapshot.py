@@ -179,6 +179,25 @@ def test_corruption_is_refused(tmp_path, world):
     assert e.value.code == A.ARC_ERR_IO
 
 
+def test_wrapping_payload_offset_is_refused(tmp_path, world):
+    """The last chunk's offset forged to an aligned value near 2^64, with both CRCs recomputed:
+    offset + payload size wraps below the file size, so only a test of each term apart refuses it."""
+    good = tmp_path / "g.arcw"
+    world.save(good)
+    data = bytearray(good.read_bytes())
+    n = HDR.unpack_from(data, 0)[6]
+    chunk, nn, nb, crc, off, solid = ENTRY.unpack_from(data, 64 + 32 * (n - 1))
+    payload = 16 * nn + 64 * nb
+    forged = 2 ** 64 - 4096  # aligned and beyond every earlier payload; offset + size wraps
+    assert payload > 4096 and forged > off and (forged + payload) % 2 ** 64 < len(data)
+    struct.pack_into("<Q", data, 64 + 32 * (n - 1) + 16, forged)
+    p = tmp_path / "wrap.arcw"
+    rewrite(p, data, fix_index=True)
+    with pytest.raises(A.ArcError) as e:
+        A.Snapshot(p)
+    assert e.value.code == A.ARC_ERR_CORRUPT and "offset" in str(e.value)
+
+
 def test_forged_tree_pointer_is_refused(tmp_path):
     """A child pointer outside the chunk's node array, with every checksum made valid again."""
     w = heightmap_with_empty_chunks()
