@@ -354,6 +354,7 @@ def _gpu_trace(ctx, pools, params, rays, full):
 def test_gpu_step_kat(ctx, name):
     pools, params, ray = case_inputs(name)
     print(f"[kat] {name}", flush=True)
+    ctx.upload(*pools)
     check_record(name, ctx.trace(params, ray)[0])
     full = _gpu_trace(ctx, pools, params, ray, True)
     ref_full, _ = O.trace_world(O.World(*pools), params, ray)
