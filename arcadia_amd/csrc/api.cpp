@@ -660,6 +660,39 @@ int arc_trace_full_device(arc_ctx* c, const arc_trace_params* params, const arc_
   return launch(c, params, d_rays, n, d_out, true, stream);
 }
 
+int arc_trace_pools_noalloc(const uint32_t grid[3], const uint32_t* d_roots, const arc_node64* d_nodes,
+                            const arc_brick64* d_bricks, const arc_trace_params* params, const arc_ray* d_rays,
+                            uint64_t n, arc_hit* d_hits, void* stream) {
+  if (!grid || !valid_params(params)) return ARC_ERR_INVALID;
+  for (int a = 0; a < 3; ++a)
+    if (grid[a] == 0 || grid[a] > 256) return ARC_ERR_INVALID;
+  if (n == 0) return ARC_OK;
+  if (!d_roots || !d_nodes || !d_bricks || !d_rays || !d_hits) return ARC_ERR_INVALID;
+  if (n > 0x7FFFFFFFull) return ARC_ERR_CAPACITY;
+  uint32_t* queue = nullptr;
+  uint32_t n_cu = 0;
+  if (arcdev::noalloc_scratch(&queue, &n_cu) != hipSuccess) return ARC_ERR_HIP;
+  const arc_launch_config cfg = default_config();
+  arcdev::LaunchArgs a{};
+  std::memcpy(a.grid, grid, sizeof a.grid);
+  a.roots = d_roots;
+  a.nodes = d_nodes;
+  a.bricks = d_bricks;
+  a.params = *params;
+  a.rays = d_rays;
+  a.out = d_hits;
+  a.n = n;
+  a.full = false;
+  a.queue = queue;
+  const uint64_t want_blocks = (uint64_t)cfg.blocks_per_cu * n_cu;
+  const uint64_t max_useful = (n + cfg.block_threads - 1) / cfg.block_threads;
+  a.blocks = (uint32_t)std::max<uint64_t>(1u, std::min(want_blocks, max_useful));
+  a.block_threads = cfg.block_threads;
+  a.refill = cfg.refill_lanes;
+  a.batch = cfg.batch_rays;
+  return arcdev::launch_trace(a, (hipStream_t)stream) == hipSuccess ? ARC_OK : ARC_ERR_HIP;
+}
+
 int arc_autotune(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n, arc_hit* d_hits,
                  void* stream, uint32_t reps, arc_launch_config* best, float* best_ms) {
   if (!c) return ARC_ERR_INVALID;

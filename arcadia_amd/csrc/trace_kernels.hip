@@ -28,6 +28,8 @@
 // counters, which keeps a launch self-contained (no memset node: graph-safe).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "raymath.h"
 #include "trace_device.h"
 #include "trace_launch.h"
@@ -431,4 +433,45 @@ hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------
+// Queue scratch of the allocation-free entry point (arc_trace_pools_noalloc)
+// ---------------------------------------------------------------------------------------
+// One queue region (kQueueRegionWords words, 32 KiB) in static device memory: each GPU the library
+// runs on holds its own copy until the process ends. The array is zero when the code object loads,
+// and the last workgroup of every launch returns the counters to zero, so consecutive calls need
+// no clearing. Calls that overlap on one GPU (two streams) share the counters and race.
+// In host code `scratch` names a host-side stand-in, never a device address: the device address
+// comes from hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) uint32_t scratch[kQueueRegionWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+uint32_t g_scratch_cu[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(uint32_t** queue, uint32_t* n_cu) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    int cu = 0;
+    e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (p == nullptr || cu <= 0) return hipErrorInvalidValue;
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+    g_scratch_cu[dev] = (uint32_t)cu;
+  }
+  *queue = g_scratch_ptr[dev];
+  *n_cu = g_scratch_cu[dev];
+  return hipSuccess;
+}
+
 }  // namespace arcdev
+

@@ -39,6 +39,10 @@ struct LaunchArgs {
 
 hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream);
 
+// The static queue scratch of the allocation-free trace on the current device and that device's
+// compute-unit count (the scratch's device address is looked up once per device).
+hipError_t noalloc_scratch(uint32_t** queue, uint32_t* n_cu);
+
 // One view of a path batch (device copy).
 struct ViewDev {
   arc_camera cam;

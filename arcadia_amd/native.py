@@ -307,6 +307,8 @@ _SIGS = [
     ("arc_nrc_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_float),
                                C.POINTER(C.c_uint32)]),
     ("arc_nrc_params_io", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("arc_trace_pools_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_uint64, C.c_void_p, C.c_void_p]),
     ("arc_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
@@ -689,6 +691,15 @@ def bounce_rays(prays: np.ndarray, hits: np.ndarray, seed: int = 1, max_dist_m: 
     if k < 0:
         raise ArcError(int(k), "bounce_rays")
     return out[:k].copy(), src[:k].copy()
+
+
+def trace_pools_noalloc(grid, d_roots: int, d_nodes: int, d_bricks: int, params: TraceParams, d_rays: int, n: int,
+                        d_hits: int, stream: int = 0) -> int:
+    """arc_trace_pools_noalloc: compact hit records of `n` rays on caller-owned device pools, with no
+    context and no allocation (device pointers, asynchronous on `stream`); returns the status code."""
+    g = (C.c_uint32 * 3)(*(int(v) for v in grid))
+    return int(lib().arc_trace_pools_noalloc(g, d_roots, d_nodes, d_bricks, C.byref(params), d_rays, n, d_hits,
+                                             stream or None))
 
 
 # ---------------------------------------------------------------------------------------------

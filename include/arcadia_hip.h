@@ -208,6 +208,19 @@ int arc_trace(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* rays,
 /* Device-pointer variant, asynchronous on `stream` (a hipStream_t; NULL = the null stream). */
 int arc_trace_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
                      uint64_t n, arc_hit* d_hits, void* stream);
+/* Allocation-free trace. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It runs arc_trace_device's kernels with the default launch shape on world pools the
+ * caller owns (the layout arc_world_upload takes: grid dims 1..256, chunk_roots, node and brick
+ * pools, already checked by the caller, for instance through arc_world_upload once), on the
+ * current HIP device, and needs no context. Its only scratch is the ray queue's claim counters, a
+ * static 32 KiB device array per GPU; nothing is allocated or freed per call. Limit: n <= 2^31-1
+ * rays (ARC_ERR_CAPACITY beyond). Calls that overlap on one GPU (different streams) share that
+ * array and race; calls in one stream order are safe. Errors: ARC_ERR_INVALID (null pointer,
+ * bad grid or params), ARC_ERR_HIP (a launch or device query failed). */
+int arc_trace_pools_noalloc(const uint32_t grid[3], const uint32_t* d_roots, const arc_node64* d_nodes,
+                            const arc_brick64* d_bricks, const arc_trace_params* params, const arc_ray* d_rays,
+                            uint64_t n, arc_hit* d_hits, void* stream);
 /* Full TraceResult records (hit position, normal, lod rgb, voxel hash, contact AO). */
 int arc_trace_full_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
                           uint64_t n, arc_trace_result* d_out, void* stream);
