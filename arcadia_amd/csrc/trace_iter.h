@@ -21,16 +21,16 @@ namespace arcdev {
 //   * the box step computes the clamped inside coordinate of all three axes and selects, with no
 //     branches (box_step);
 //   * WATER = false compiles the ignore_water tests out (ray classes that never set it).
-template <uint32_t SEM, bool WATER>
-__device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C, const RootLds& R,
-                                          bool lds_roots) {
+template <uint32_t SEM, bool WATER, uint32_t BLOCK = kMaxBlock, uint32_t NCH = kLdsChunks>
+__device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C,
+                                          const RootLdsT<NCH>& R, bool lds_roots) {
   uint32_t st = L.st;
   uint4 node = L.node;
   uint32_t sh = fget(st, F_CSHIFT, 3);
   const uint32_t sidx = L.step;
   L.step = sidx + 1u;
   const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
-  const bool descend = restart<SEM>(w, L, C, R, lds_roots, minsh, sh, node);
+  const bool descend = restart<SEM, BLOCK, NCH>(w, L, C, R, lds_roots, minsh, sh, node);
   uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh (empty chunk: 256)
   bool cand = false;  // set bit at a leaf or at the LOD floor: hit unless the voxel is air/water
   if (descend) {
@@ -54,7 +54,7 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       if (go) {
         node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
         sh -= 2u;
-        *cold_slot(C, sh) = node;
+        *cold_slot<BLOCK>(C, sh) = node;
       }
     }
   }

@@ -301,11 +301,16 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
 }
 
 // `queue`: one queue region (claim counters and finished workgroups, zero between launches).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
+// Two shapes: workgroups of 256 lanes with the roots of worlds up to 256 chunks in LDS (21 KB per
+// workgroup), or one workgroup of 1024 lanes per CU holding the roots of worlds up to 4096 chunks
+// (80 KB) beside its 64 KB of cold slots; both keep the 4 waves per SIMD the kernel's registers
+// allow.
+template <uint32_t BLOCK, uint32_t NCH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
     DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
     uint32_t refill, uint32_t batch) {
-  __shared__ uint4 s_cold[kColdSlots * kMaxBlock];
-  __shared__ RootLds s_roots;
+  __shared__ uint4 s_cold[kColdSlots * BLOCK];
+  __shared__ RootLdsT<NCH> s_roots;
   uint4* const C = s_cold + threadIdx.x;
   const bool lds_roots = stage_roots(w, s_roots);
   const uint32_t lane = __lane_id();
@@ -348,7 +353,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
     // straight on; the exit test above runs only when nothing was pending.
     if (fget(L.st, F_LIVE, 1))
-      iteration<ARC_SEM_WORLD, true>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots, lds_roots);
+      iteration<ARC_SEM_WORLD, true, BLOCK, NCH>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots,
+                                                 lds_roots);
   }
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
@@ -410,10 +416,14 @@ hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+static const void* path_kernel(uint32_t block_threads) {
+  return block_threads == kPathLargeBlock ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge>)
+                                          : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks>);
+}
+
 hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {
   int nb = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&k_path),
-                                                               (int)block_threads, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, path_kernel(block_threads), (int)block_threads, 0);
   *blocks_per_cu = nb > 0 ? (uint32_t)nb : 1u;
   return e;
 }
@@ -428,8 +438,14 @@ hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream) {
   pc.n_bounces = a.n_bounces;
   pc.bounce_tmax = a.bounce_max_dist_m;
   pc.seed = a.seed;
-  hipLaunchKernelGGL(k_path, dim3(a.blocks), dim3(a.block_threads), 0, stream, w, make_params(a.primary),
-                     make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue, a.refill, a.batch);
+  if (a.block_threads == kPathLargeBlock)
+    hipLaunchKernelGGL((k_path<kPathLargeBlock, kLdsChunksLarge>), dim3(a.blocks), dim3(a.block_threads), 0, stream,
+                       w, make_params(a.primary), make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue,
+                       a.refill, a.batch);
+  else
+    hipLaunchKernelGGL((k_path<kMaxBlock, kLdsChunks>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                       make_params(a.primary), make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue,
+                       a.refill, a.batch);
   return hipGetLastError();
 }
 

@@ -66,6 +66,13 @@ struct PathLaunchArgs {
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream);
+// Chunk roots staged in LDS (root index + root node, 20 B per chunk): kLdsChunks covers the
+// 4x4x4-chunk worlds in 5 KB per workgroup of 256 lanes; kLdsChunksLarge covers a 16x16x16-chunk
+// world in 80 KB, held by the path kernel's large shape, one workgroup of kPathLargeBlock lanes
+// per CU.
+constexpr uint32_t kLdsChunks = 256u;
+constexpr uint32_t kLdsChunksLarge = 4096u;
+constexpr uint32_t kPathLargeBlock = 1024u;
 
 // One launch of a shadow any-hit walker over a ray set.
 struct ShadowLaunchArgs {

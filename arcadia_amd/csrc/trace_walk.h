@@ -147,24 +147,29 @@ __device__ __forceinline__ uint32_t restart_shift(uint32_t diff, uint32_t minsh,
   return umax_(dl, umax_(minsh, cs));
 }
 
-// Address of the LDS slot of the cached node at shift `sh` (slot (6 - sh) / 2).
+// Address of the LDS slot of the cached node at shift `sh` (slot (6 - sh) / 2) in a workgroup of
+// BLOCK lanes.
+template <uint32_t BLOCK = kMaxBlock>
 __device__ __forceinline__ uint4* cold_slot(uint4* C, uint32_t sh) {
-  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (kMaxBlock * 8u));
+  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (BLOCK * 8u));
 }
 
-// Chunk roots staged in LDS for worlds of at most kLdsChunks chunks: the root index and the root
-// node of every chunk, read at each chunk change in place of chunk_roots[key] followed by
+// Chunk roots staged in LDS for worlds of at most N chunks: the root index and the root node of
+// every chunk (20 B per chunk), read at each chunk change in place of chunk_roots[key] followed by
 // nodes[root] (two dependent global loads on the longest dependency chain of a wave trip).
-constexpr uint32_t kLdsChunks = 256u;
-struct RootLds {
-  uint4 node[kLdsChunks];
-  uint32_t idx[kLdsChunks];
+// kLdsChunks / kLdsChunksLarge (trace_launch.h) are the two capacities in use.
+template <uint32_t N>
+struct RootLdsT {
+  uint4 node[N];
+  uint32_t idx[N];
 };
+using RootLds = RootLdsT<kLdsChunks>;
 
 // Fills `R` from the pools; returns false (block-uniform) when the world has more chunks.
-__device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLds& R) {
+template <uint32_t N>
+__device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLdsT<N>& R) {
   const uint32_t nc = w.gx * w.gy * w.gz;
-  if (nc > kLdsChunks) return false;
+  if (nc > N) return false;
   for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
     const uint32_t r = w.roots[c];
     R.idx[c] = r;
@@ -183,11 +188,11 @@ __device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLds& R) {
 // The cached-ancestor read runs for every lane; a chunk change (restart_shift gives 6 there) then
 // replaces the node it read (the previous chunk's root) with the new chunk's root. Two plain
 // branches, the second skipped by whole waves, instead of an if / else.
-template <uint32_t SEM>
-__device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4* C, const RootLds& R, bool lds_roots,
-                                        uint32_t minsh, uint32_t& sh, uint4& node) {
+template <uint32_t SEM, uint32_t BLOCK = kMaxBlock, uint32_t NCH = kLdsChunks>
+__device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4* C, const RootLdsT<NCH>& R,
+                                        bool lds_roots, uint32_t minsh, uint32_t& sh, uint4& node) {
   const uint32_t rs = restart_shift(L.diff, minsh, sh);
-  if (rs != sh) node = *cold_slot(C, rs);
+  if (rs != sh) node = *cold_slot<BLOCK>(C, rs);
   sh = rs;
   bool ok = true;
   if (SEM == ARC_SEM_WORLD && L.diff >= 256u) {

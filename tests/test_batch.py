@@ -144,7 +144,8 @@ def test_gpu_batch_bit_exact_and_graph_replays(ctx):
         torch.cuda.synchronize()
         assert len(W.diff_records(g.cpu().numpy().view(A.HIT_DTYPE), ref)) == 0
         assert len(W.diff_records(e.cpu().numpy().view(A.HIT_DTYPE), ref)) == 0
-    for cfg in ({"block_threads": 128, "refill_lanes": 8, "batch_rays": 32}, {"blocks_per_cu": 2, "refill_lanes": 48}):
+    for cfg in ({"block_threads": 128, "refill_lanes": 8, "batch_rays": 32}, {"blocks_per_cu": 2, "refill_lanes": 48},
+                {"block_threads": 1024, "refill_lanes": 24}):  # 1024: one workgroup per CU, every chunk root in LDS
         b.set_launch_config(**cfg)
         e.zero_()
         b.trace(e.data_ptr())
