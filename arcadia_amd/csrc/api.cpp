@@ -34,6 +34,9 @@ struct arc_ctx {
   arc_brick64* d_bricks = nullptr;
   uint64_t n_nodes = 0, node_cap = 0, n_bricks = 0, brick_cap = 0;
   bool has_world = false;
+  // The pools may hold water (ARC_MAT_WATER as a node's lod material or a brick byte). When they
+  // cannot, ignore_water changes no result and traces run the kernels without the water tests.
+  bool water = true;
   // persistent queues (kQueueRegionWords words each): the traversal kernel's, the path kernel's,
   // then the shadow walkers'
   uint32_t* d_queue = nullptr;
@@ -274,6 +277,7 @@ int arc_world_upload(arc_ctx* c, const uint32_t grid[3], const uint32_t* roots, 
   c->node_cap = node_cap;
   c->n_bricks = n_bricks;
   c->brick_cap = brick_cap;
+  c->water = arc::pools_hold_material(nodes, n_nodes, bricks, n_bricks, ARC_MAT_WATER);
   c->has_world = true;
   c->world_gen++;
   return ARC_OK;
@@ -368,6 +372,7 @@ int arc_snapshot_upload(arc_ctx* c, const arc_snapshot* sh, uint64_t node_cap, u
   }
   const hipStream_t stream = nullptr;
   const int nt = std::max(1, std::min(nthreads, 64));
+  std::atomic<bool> water{false};  // any staged chunk holds water
   for (size_t bi = 0; bi < batches.size(); ++bi) {
     const Batch& b = batches[bi];
     const int slot = (int)(bi & 1);
@@ -388,6 +393,10 @@ int arc_snapshot_upload(arc_ctx* c, const arc_snapshot* sh, uint64_t node_cap, u
         const uint8_t* p = arc::snap_payload(s, k);
         std::memcpy(nodes_h + (node_at[k] - b.node0) * 16, p, (size_t)e.n_nodes * 16);
         std::memcpy(bricks_h + (brick_at[k] - b.brick0) * 64, p + (size_t)e.n_nodes * 16, (size_t)e.n_bricks * 64);
+        if (arc::pools_hold_material(reinterpret_cast<const arc_node64*>(p), e.n_nodes,
+                                     reinterpret_cast<const arc_brick64*>(p + (size_t)e.n_nodes * 16), e.n_bricks,
+                                     ARC_MAT_WATER))
+          water.store(true);
         const uint4 t = make_uint4((uint32_t)node_at[k], e.n_nodes, (uint32_t)brick_at[k], 0u);
         std::memcpy(tab + (size_t)(k - b.k0) * sizeof(uint4), &t, sizeof t);
       }
@@ -430,6 +439,7 @@ int arc_snapshot_upload(arc_ctx* c, const arc_snapshot* sh, uint64_t node_cap, u
   c->node_cap = node_cap;
   c->n_bricks = n_bricks;
   c->brick_cap = brick_cap;
+  c->water = water.load();
   c->has_world = true;
   c->world_gen++;
   return ARC_OK;
@@ -472,6 +482,12 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
   std::string err;
   const int rc = arcplan::plan_patches(patches, n, nchunks, c->node_cap, c->brick_cap, plan, err);
   if (rc != ARC_OK) return fail(c, rc, "%s", err.c_str());
+  for (uint64_t i = 0; i < n && !c->water; ++i)
+    if (arc::pools_hold_material(patches[i].nodes, patches[i].n_nodes, patches[i].bricks, patches[i].n_bricks,
+                                 ARC_MAT_WATER)) {
+      c->water = true;
+      c->world_gen++;  // captured batches were recorded without the water tests: capture again
+    }
   HIP_TRY(c, hipSetDevice(c->device));
   // Work queued on any stream may still read the pools.
   HIP_TRY(c, hipDeviceSynchronize());
@@ -542,6 +558,7 @@ static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_r
   a.nodes = c->d_nodes;
   a.bricks = c->d_bricks;
   a.params = *params;
+  if (!c->water) a.params.ignore_water = 0u;  // no water in the pools: the same results without its tests
   a.rays = d_rays;
   a.out = d_out;
   a.n = n;
@@ -860,6 +877,7 @@ int batch_launch(arc_batch* b, arc_hit* d_hits, hipStream_t s) {
   a.bricks = c->d_bricks;
   a.primary = b->primary;
   a.bounce = b->bounce;
+  if (!c->water) a.primary.ignore_water = a.bounce.ignore_water = 0u;  // as in launch()
   a.views = b->d_views;
   a.view_first = b->d_first;
   a.n_views = b->n_views;

@@ -60,6 +60,14 @@ uint32_t crc32(const void* data, size_t n, uint32_t crc) {
   return ~c;
 }
 
+bool pools_hold_material(const arc_node64* nodes, uint64_t n_nodes, const arc_brick64* bricks, uint64_t n_bricks,
+                         uint32_t mat) {
+  for (uint64_t i = 0; i < n_nodes; ++i)
+    if ((nodes[i].lod_payload >> 24) == mat) return true;
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(bricks);
+  return n_bricks != 0 && std::memchr(b, (int)mat, (size_t)n_bricks * sizeof(arc_brick64)) != nullptr;
+}
+
 // Structural check of a tree: every reachable node's children lie inside the node array, mixed
 // leaves point inside the brick array, and the tree is at most four levels deep with leaves at
 // the last level (the invariants the traversal kernels rely on).

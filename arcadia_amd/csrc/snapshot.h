@@ -81,4 +81,11 @@ int snap_check_chunk(const Snapshot& s, uint32_t k, std::string& why);
 // The tree rules of the upload check, over one tree rooted at `root`.
 int check_tree(const arc_node64* nodes, uint64_t n_nodes, uint64_t n_bricks, uint32_t root, std::string& why);
 
+// Whether any node's lod_payload material or any brick byte is `mat`. The traversal reads a
+// material from exactly those two places (lod_payload >> 24 of uniform nodes and LOD stops, brick
+// bytes of mixed leaves), so pools where this is false for water trace identically with and
+// without ignore_water.
+bool pools_hold_material(const arc_node64* nodes, uint64_t n_nodes, const arc_brick64* bricks, uint64_t n_bricks,
+                         uint32_t mat);
+
 }  // namespace arc
