@@ -305,7 +305,8 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
 // workgroup), or one workgroup of 1024 lanes per CU holding the roots of worlds up to 4096 chunks
 // (80 KB) beside its 64 KB of cold slots; both keep the 4 waves per SIMD the kernel's registers
 // allow.
-template <uint32_t BLOCK, uint32_t NCH>
+// WATER = false compiles the ignore_water tests out (neither ray class sets it).
+template <uint32_t BLOCK, uint32_t NCH, bool WATER>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
     DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
     uint32_t refill, uint32_t batch) {
@@ -353,8 +354,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
     // straight on; the exit test above runs only when nothing was pending.
     if (fget(L.st, F_LIVE, 1))
-      iteration<ARC_SEM_WORLD, true, BLOCK, NCH>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots,
-                                                 lds_roots);
+      iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots,
+                                                  lds_roots);
   }
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
@@ -417,8 +418,16 @@ hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
 }
 
 static const void* path_kernel(uint32_t block_threads) {
-  return block_threads == kPathLargeBlock ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge>)
-                                          : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks>);
+  return block_threads == kPathLargeBlock
+             ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge, true>)
+             : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks, true>);
+}
+
+template <uint32_t BLOCK, uint32_t NCH, bool WATER>
+static void launch_path_as(const PathLaunchArgs& a, const DevWorld& w, const PathCtl& pc, hipStream_t stream) {
+  hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                     make_params(a.primary), make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue,
+                     a.refill, a.batch);
 }
 
 hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {
@@ -438,14 +447,12 @@ hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream) {
   pc.n_bounces = a.n_bounces;
   pc.bounce_tmax = a.bounce_max_dist_m;
   pc.seed = a.seed;
-  if (a.block_threads == kPathLargeBlock)
-    hipLaunchKernelGGL((k_path<kPathLargeBlock, kLdsChunksLarge>), dim3(a.blocks), dim3(a.block_threads), 0, stream,
-                       w, make_params(a.primary), make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue,
-                       a.refill, a.batch);
-  else
-    hipLaunchKernelGGL((k_path<kMaxBlock, kLdsChunks>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
-                       make_params(a.primary), make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue,
-                       a.refill, a.batch);
+  const bool water = a.primary.ignore_water || a.bounce.ignore_water;
+  const bool large = a.block_threads == kPathLargeBlock;
+  if (large && water) launch_path_as<kPathLargeBlock, kLdsChunksLarge, true>(a, w, pc, stream);
+  else if (large) launch_path_as<kPathLargeBlock, kLdsChunksLarge, false>(a, w, pc, stream);
+  else if (water) launch_path_as<kMaxBlock, kLdsChunks, true>(a, w, pc, stream);
+  else launch_path_as<kMaxBlock, kLdsChunks, false>(a, w, pc, stream);
   return hipGetLastError();
 }
 
