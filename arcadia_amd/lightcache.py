@@ -62,14 +62,9 @@ class CacheRun:
         p0, pb = N.TraceParams.exact(0), N.TraceParams.exact(2, ignore_water=True)
         self.eval = ctx.batch([(cam, width, height, max_dist_m)], p0, pb, n_bounces=4, bounce_max_dist_m=max_dist_m,
                               seed=seed)
-        tw, th = train_size
-        tcam = N.camera_look((0.0, 0.0, 0.0), 0.0, 0.0, 60.0, tw, th)  # same basis, train resolution
-        for f in ("eye", "forward", "right", "up"):
-            setattr(tcam, f, getattr(cam, f))
-        tcam.tan_half_fov = cam.tan_half_fov
-        tcam.aspect = tw / th
-        self.train = ctx.batch([(tcam, tw, th, max_dist_m)], p0, pb, n_bounces=4, bounce_max_dist_m=max_dist_m,
-                               seed=seed)
+        self.max_dist_m = max_dist_m
+        self.seed = seed
+        self.train = self.train_batch(train_size)
         dev = "cuda"
         ne, nt = self.eval.slots, self.train.slots
         self.n_eval, self.n_train = ne, nt
@@ -85,6 +80,18 @@ class CacheRun:
         self.sharc = N.Sharc(sharc_log2)
         self.nrc = N.Nrc(nrc or N.NrcParams.default())
         self.seed = seed
+
+    def train_batch(self, size):
+        """One 4-bounce path per pixel of a `size` view of the camera (same basis and vertical FOV)."""
+        tw, th = size
+        tcam = N.camera_look((0.0, 0.0, 0.0), 0.0, 0.0, 60.0, tw, th)
+        for f in ("eye", "forward", "right", "up"):
+            setattr(tcam, f, getattr(self.cam, f))
+        tcam.tan_half_fov = self.cam.tan_half_fov
+        tcam.aspect = tw / th
+        p0, pb = N.TraceParams.exact(0), N.TraceParams.exact(2, ignore_water=True)
+        return self.ctx.batch([(tcam, tw, th, self.max_dist_m)], p0, pb, n_bounces=4,
+                              bounce_max_dist_m=self.max_dist_m, seed=self.seed)
 
     def close(self):
         self.sharc.close()
@@ -158,6 +165,70 @@ class CacheRun:
         N.estimate(self.eval_samples.data_ptr(), self.n_eval, self.cache_out.data_ptr(), 1, self.est.data_ptr())
         out["nrc"] = self.rel_mse(self.est)
         return out
+
+    def train_trace_ms(self, batch=None, reps: int = 5) -> float:
+        """GPU ms to trace and shade one frame's training paths (median of `reps`)."""
+        torch = self.torch
+        b = batch or self.train
+        hits = torch.empty(b.records() * 16, dtype=torch.uint8, device="cuda")
+        samples = torch.empty(b.slots * 96, dtype=torch.uint8, device="cuda")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        t = []
+        for r in range(reps + 1):
+            ev[0].record()
+            b.trace(hits.data_ptr())
+            b.shade(hits.data_ptr(), self.shade, d_samples=samples.data_ptr())
+            ev[1].record()
+            torch.cuda.synchronize()
+            if r:
+                t.append(ev[0].elapsed_time(ev[1]))
+        return float(np.median(t))
+
+    def sharc_at_budget(self, budget_ms: float, frames: int, sharc_log2: int, eval_seed: int,
+                        trace_ms_per_path: float, sharc_ms_base: float) -> dict:
+        """SHARC given a per-frame GPU budget: as many training paths as it can trace and shade in
+        `budget_ms` beside its own work (deposit + resolve + 1080p query, `sharc_ms_base`), then
+        `frames` online frames with that many paths, judged on the same evaluation paths. The
+        measured per-frame ms (tracing, shading, deposit, resolve, query) is returned with it."""
+        torch = self.torch
+
+        def view_for(n):
+            w = max(8, int(round(np.sqrt(n * 16.0 / 9.0) / 8.0)) * 8)
+            return w, max(8, int(round(w * 9.0 / 16.0 / 8.0)) * 8)
+
+        # tracing cost per path falls as the batch grows (more waves in flight): size the batch from
+        # the small batch's cost, measure it, and rescale once with the measured cost
+        n_paths = max(self.n_train, int((budget_ms - sharc_ms_base) / trace_ms_per_path))
+        probe = self.train_batch(view_for(n_paths))
+        t_probe = self.train_trace_ms(probe, reps=3)
+        n_paths = max(self.n_train, int(probe.slots * (budget_ms - sharc_ms_base) / max(t_probe, 1e-6)))
+        probe.close()
+        tw, th = view_for(n_paths)
+        b = self.train_batch((tw, th))
+        hits = torch.empty(b.records() * 16, dtype=torch.uint8, device="cuda")
+        samples = torch.empty(b.slots * 96, dtype=torch.uint8, device="cuda")
+        sharc = N.Sharc(sharc_log2)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        t = []
+        for f in range(frames):
+            b.set_seed(self.seed * 1000003 + f)
+            ev[0].record()
+            b.trace(hits.data_ptr())
+            b.shade(hits.data_ptr(), self.shade, d_samples=samples.data_ptr())
+            sharc.deposit_paths(samples.data_ptr(), b.slots, self.cam_pos)
+            sharc.resolve()
+            sharc.query_paths(self.eval_samples.data_ptr(), self.n_eval, self.cam_pos, self.cache_out.data_ptr())
+            ev[1].record()
+            torch.cuda.synchronize()
+            t.append(ev[0].elapsed_time(ev[1]))
+        self.eval_paths(eval_seed)
+        sharc.query_paths(self.eval_samples.data_ptr(), self.n_eval, self.cam_pos, self.cache_out.data_ptr())
+        N.estimate(self.eval_samples.data_ptr(), self.n_eval, self.cache_out.data_ptr(), 1, self.est.data_ptr())
+        err = self.rel_mse(self.est)
+        sharc.close()
+        b.close()
+        return {"paths_per_frame": int(b.slots), "ms_per_frame": float(np.median(t[1:] if len(t) > 1 else t)),
+                "rel_mse": err}
 
     def cache_ms(self, reps: int = 5, nrc_steps: int = 1) -> dict:
         """GPU ms per frame of each cache's own work (training-path tracing and evaluation tracing
