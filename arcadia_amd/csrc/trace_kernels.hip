@@ -155,9 +155,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
-    if (drained) {
-      if (nidle == 64u) break;
-    } else if (nidle >= refill) {
+    // The exit test goes through readfirstlane so the compiler sees a uniform loop exit: with an
+    // exit it takes for divergent, it keeps per-lane exit masks and copies every loop-carried
+    // lane register twice per wave trip (14 extra vector moves per trip).
+    if (rfl((uint32_t)(drained && nidle == 64u))) break;
+    if (!drained && nidle >= refill) {
       // finalize finished lanes together, then refill every idle lane
       if (idle_lane && fget(L.st, F_HASRAY, 1)) {
         finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
@@ -328,6 +330,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
     const bool pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1)) != 0ull;
+    // drained, nothing pending, every lane idle; a uniform exit for the compiler (as in k_trace)
+    if (rfl((uint32_t)(drained && !pending && nidle == 64u))) break;
     if (drained ? pending : nidle >= refill) {
       if (idle_lane && fget(L.st, F_HASRAY, 1)) path_finish(w, pb, pc, out, L, tmax, C);
       if (!drained) {
@@ -348,11 +352,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if (my != 0xFFFFFFFFu) path_start(w, p0, pc, my, out, L, tmax, C);
       }
-    } else if (drained && nidle == 64u) {
-      break;  // drained, nothing pending, every lane idle
     }
     // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
-    // straight on; the exit test above runs only when nothing was pending.
+    // straight on.
     if (fget(L.st, F_LIVE, 1))
       iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots,
                                                   lds_roots);
