@@ -544,6 +544,12 @@ int arc_world_patch(arc_ctx* c, const arc_gpu_patch* patches, uint64_t n) {
   return ARC_OK;
 }
 
+// The device pools (at their capacities, which patches fill up to) each span at most 4 GiB: the
+// traversal kernels then address them with 32-bit byte offsets.
+static bool pools_off32(const arc_ctx* c) {
+  return c->node_cap * sizeof(arc_node64) <= (1ull << 32) && c->brick_cap * sizeof(arc_brick64) <= (1ull << 32);
+}
+
 static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n, void* d_out, bool full,
                   void* stream) {
   if (!c) return ARC_ERR_INVALID;
@@ -563,6 +569,7 @@ static int launch(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_r
   a.out = d_out;
   a.n = n;
   a.full = full;
+  a.off32 = pools_off32(c);
   a.queue = c->d_queue;
   const arc_launch_config& cfg = c->cfg[lod_class(params)];
   uint32_t bpc = cfg.blocks_per_cu;
@@ -886,6 +893,7 @@ int batch_launch(arc_batch* b, arc_hit* d_hits, hipStream_t s) {
   a.bounce_max_dist_m = b->bounce_tmax;
   a.seed = b->seed;
   a.out = d_hits;
+  a.off32 = pools_off32(c);
   a.queue = c->d_queue + arcdev::kQueueRegionWords;  // the path kernel's own queue
   uint32_t bpc = b->cfg.blocks_per_cu;
   if (bpc == 0) {

@@ -21,7 +21,7 @@ namespace arcdev {
 //   * the box step computes the clamped inside coordinate of all three axes and selects, with no
 //     branches (box_step);
 //   * WATER = false compiles the ignore_water tests out (ray classes that never set it).
-template <uint32_t SEM, bool WATER, uint32_t BLOCK = kMaxBlock, uint32_t NCH = kLdsChunks>
+template <uint32_t SEM, bool WATER, uint32_t BLOCK = kMaxBlock, uint32_t NCH = kLdsChunks, bool OFF32 = false>
 __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p, Lane& L, uint4* C,
                                           const RootLdsT<NCH>& R, bool lds_roots) {
   uint32_t st = L.st;
@@ -30,40 +30,46 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   const uint32_t sidx = L.step;
   L.step = sidx + 1u;
   const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
-  const bool descend = restart<SEM, BLOCK, NCH>(w, L, C, R, lds_roots, minsh, sh, node);
+  const bool descend = restart<SEM, BLOCK, NCH, OFF32>(w, L, C, R, lds_roots, minsh, sh, node);
   uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh (empty chunk: 256)
-  bool cand = false;  // set bit at a leaf or at the LOD floor: hit unless the voxel is air/water
+  // Sign bit of `hw`: the voxel's child bit is set at the level the descent ended on (a hit
+  // candidate: hit unless the voxel is air/water). `b` is that child's slot.
+  uint32_t hw = 0u, b = 0u;
   if (descend) {
     // descent (:390-552): at most three loads (shift 6 -> 4 -> 2 -> 0; shift 0 always stops), as
     // four unrolled tests whose exit is wave-uniform, so the compiler keeps no per-lane loop-exit
     // masks. A lane that has stopped repeats its own test with unchanged state until the wave exits.
+    // The tests are sign bits combined in one word, so `go` is a single compare (the ballot folds
+    // into it): bit 31 of the high word of `below` = child present; bit 31 of node.z = leaf;
+    // bit 31 of minsh - sh = sh > minsh (both below 8).
 #pragma unroll
     for (uint32_t pass = 0; pass < 4u; ++pass) {
-      const uint32_t b = child_bit(L, sh);
-      const bool water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
-                         (node.w >> 24) == ARC_MAT_WATER;
+      b = child_bit(L, sh);
       const uint64_t below = (((uint64_t)node.y << 32) | node.x) << (63u - b);
-      const bool has = (int64_t)below < 0 && !water;  // child at b
+      hw = (uint32_t)(below >> 32);
+      if (SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
+          (node.w >> 24) == ARC_MAT_WATER) {
+        hw = 0u;  // a uniform water node ignored: no child here, step over the whole cell
+        span_mul = 4u;
+      }
       // Level-3 nodes are leaves in every pool the upload check accepts; sh == 0 (implied by
       // sh <= minsh) only bounds the descent if a malformed node slipped in through a patch.
-      const bool stop = (node.z & ARC_IS_LEAF_FLAG) != 0u || sh <= minsh;
-      if (water) span_mul = 4u;
-      cand = has && stop;
-      const bool go = has && !stop;
+      const bool go = (int32_t)(hw & ~node.z & (minsh - sh)) < 0;
       if (__builtin_amdgcn_ballot_w64(go) == 0ull) break;
       if (go) {
-        node = w.nodes[(node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u];
+        node = pool_node<OFF32>(w, (node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u);
         sh -= 2u;
         *cold_slot<BLOCK>(C, sh) = node;
       }
     }
   }
+  // After the loop no lane goes on: a set child bit is at a leaf or at the LOD floor.
+  const bool cand = (int32_t)hw < 0;
   bool hit = false;
   if (cand) {  // :441-538
-    const uint32_t b = child_bit(L, sh);
     uint32_t mat = node.w >> 24;
     if ((node.z & (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG)) == ARC_IS_LEAF_FLAG) {
-      const uint32_t word = w.bricks[(size_t)(node.z & ARC_PTR_INDEX_MASK) * 16u + (b >> 2)];
+      const uint32_t word = pool_brick_word<OFF32>(w, node.z & ARC_PTR_INDEX_MASK, b >> 2);
       mat = __builtin_amdgcn_ubfe(word, (b & 3u) * 8u, 8);
     }
     const bool skip_water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && mat == ARC_MAT_WATER;

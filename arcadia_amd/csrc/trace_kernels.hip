@@ -132,7 +132,7 @@ __device__ __forceinline__ void finalize(typename OutPtr<FULL>::T* out, const ar
 // `queue`: one queue region (claim counters and finished workgroups, zero between launches).
 // Occupancy target: the compact variants fit 64 VGPRs (8 waves/SIMD) without spilling; the
 // full-record variants carry more live values through finalize and keep the default budget.
-template <uint32_t SEM, bool FULL, bool WATER>
+template <uint32_t SEM, bool FULL, bool WATER, bool OFF32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 6, 8))) void k_trace(
     DevWorld w, DevParams p, const arc_ray* __restrict__ rays,
                                                typename OutPtr<FULL>::T* __restrict__ out, uint32_t n,
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
     // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
     // `continue` back to the ballot made a second back-edge, and the register allocator then
     // copied the whole lane state on every wave trip.)
-    if (fget(L.st, F_LIVE, 1)) iteration<SEM, WATER>(w, p, L, C, s_roots, lds_roots);
+    if (fget(L.st, F_LIVE, 1)) iteration<SEM, WATER, kMaxBlock, kLdsChunks, OFF32>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
@@ -308,7 +308,7 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
 // (80 KB) beside its 64 KB of cold slots; both keep the 4 waves per SIMD the kernel's registers
 // allow.
 // WATER = false compiles the ignore_water tests out (neither ray class sets it).
-template <uint32_t BLOCK, uint32_t NCH, bool WATER>
+template <uint32_t BLOCK, uint32_t NCH, bool WATER, bool OFF32>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
     DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
     uint32_t refill, uint32_t batch) {
@@ -356,8 +356,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
     // straight on.
     if (fget(L.st, F_LIVE, 1))
-      iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C, s_roots,
-                                                  lds_roots);
+      iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C,
+                                                         s_roots, lds_roots);
   }
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
@@ -385,7 +385,7 @@ static DevParams make_params(const arc_trace_params& t) {
 
 template <uint32_t SEM, bool FULL>
 static const void* kernel_ptr() {
-  return reinterpret_cast<const void*>(&k_trace<SEM, FULL, SEM == ARC_SEM_WORLD>);
+  return reinterpret_cast<const void*>(&k_trace<SEM, FULL, SEM == ARC_SEM_WORLD, true>);
 }
 
 hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_threads, uint32_t* blocks_per_cu) {
@@ -399,8 +399,13 @@ hipError_t kernel_occupancy(uint32_t semantics, bool full, uint32_t block_thread
 
 template <uint32_t SEM, bool FULL, bool WATER>
 static void launch_one(const LaunchArgs& a, const DevWorld& w, const DevParams& p, hipStream_t stream) {
-  hipLaunchKernelGGL((k_trace<SEM, FULL, WATER>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p, a.rays,
-                     reinterpret_cast<typename OutPtr<FULL>::T*>(a.out), (uint32_t)a.n, a.queue, a.refill, a.batch);
+  auto out = reinterpret_cast<typename OutPtr<FULL>::T*>(a.out);
+  if (a.off32)
+    hipLaunchKernelGGL((k_trace<SEM, FULL, WATER, true>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p,
+                       a.rays, out, (uint32_t)a.n, a.queue, a.refill, a.batch);
+  else
+    hipLaunchKernelGGL((k_trace<SEM, FULL, WATER, false>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p,
+                       a.rays, out, (uint32_t)a.n, a.queue, a.refill, a.batch);
 }
 
 hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
@@ -421,15 +426,20 @@ hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
 
 static const void* path_kernel(uint32_t block_threads) {
   return block_threads == kPathLargeBlock
-             ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge, true>)
-             : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks, true>);
+             ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge, true, true>)
+             : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks, true, true>);
 }
 
 template <uint32_t BLOCK, uint32_t NCH, bool WATER>
 static void launch_path_as(const PathLaunchArgs& a, const DevWorld& w, const PathCtl& pc, hipStream_t stream) {
-  hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
-                     make_params(a.primary), make_params(a.bounce), pc, reinterpret_cast<uint4*>(a.out), a.queue,
-                     a.refill, a.batch);
+  const DevParams p0 = make_params(a.primary), pb = make_params(a.bounce);
+  uint4* const out = reinterpret_cast<uint4*>(a.out);
+  if (a.off32)
+    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, true>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p0, pb,
+                       pc, out, a.queue, a.refill, a.batch);
+  else
+    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, false>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p0, pb,
+                       pc, out, a.queue, a.refill, a.batch);
 }
 
 hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {

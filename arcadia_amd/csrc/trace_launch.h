@@ -30,6 +30,7 @@ struct LaunchArgs {
   void* out;
   uint64_t n;
   bool full;
+  bool off32;       // node and brick pools each within 4 GiB (see pool_node)
   uint32_t* queue;  // kQueueRegionWords u32 words, zero between launches
   uint32_t blocks;
   uint32_t block_threads;
@@ -62,6 +63,7 @@ struct PathLaunchArgs {
   float bounce_max_dist_m;
   uint64_t seed;
   arc_hit* out;      // (1 + n_bounces) * n_items records
+  bool off32;        // node and brick pools each within 4 GiB (see pool_node)
   uint32_t* queue;   // kQueueRegionWords u32 words, zero between launches
   uint32_t blocks, block_threads, refill, batch;
 };

@@ -154,6 +154,24 @@ __device__ __forceinline__ uint4* cold_slot(uint4* C, uint32_t sh) {
   return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (BLOCK * 8u));
 }
 
+// Pool reads of the traversal loop. OFF32: the node pool spans at most 4 GiB (2^28 nodes) and the
+// brick pool at most 4 GiB (2^26 bricks), so an element's byte offset fits 32 bits and the load
+// takes the scalar-base + 32-bit vector-offset form: one shift of address arithmetic instead of
+// a 64-bit multiply-add and the move that builds its high word. The launchers pick OFF32 from the
+// pool capacities; without them (pools passed as bare pointers) the 64-bit form is used.
+template <bool OFF32>
+__device__ __forceinline__ uint4 pool_node(const DevWorld& w, uint32_t i) {
+  if (!OFF32) return w.nodes[i];
+  return *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(w.nodes) + (uint32_t)(i << 4));
+}
+
+// Word `wd` (0..15) of brick `i`.
+template <bool OFF32>
+__device__ __forceinline__ uint32_t pool_brick_word(const DevWorld& w, uint32_t i, uint32_t wd) {
+  if (!OFF32) return w.bricks[(size_t)i * 16u + wd];
+  return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(w.bricks) + (uint32_t)((i << 6) | (wd << 2)));
+}
+
 // Chunk roots staged in LDS for worlds of at most N chunks: the root index and the root node of
 // every chunk (20 B per chunk), read at each chunk change in place of chunk_roots[key] followed by
 // nodes[root] (two dependent global loads on the longest dependency chain of a wave trip).
@@ -188,7 +206,7 @@ __device__ __forceinline__ bool stage_roots(const DevWorld& w, RootLdsT<N>& R) {
 // The cached-ancestor read runs for every lane; a chunk change (restart_shift gives 6 there) then
 // replaces the node it read (the previous chunk's root) with the new chunk's root. Two plain
 // branches, the second skipped by whole waves, instead of an if / else.
-template <uint32_t SEM, uint32_t BLOCK = kMaxBlock, uint32_t NCH = kLdsChunks>
+template <uint32_t SEM, uint32_t BLOCK = kMaxBlock, uint32_t NCH = kLdsChunks, bool OFF32 = false>
 __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4* C, const RootLdsT<NCH>& R,
                                         bool lds_roots, uint32_t minsh, uint32_t& sh, uint4& node) {
   const uint32_t rs = restart_shift(L.diff, minsh, sh);
@@ -199,11 +217,14 @@ __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4*
     const uint32_t key = (L.vx >> 8) + __umul24(L.vy >> 8, w.gx) + __umul24(L.vz >> 8, w.gx * w.gy);
     uint32_t root;
     if (lds_roots) {
-      root = R.idx[key];
-      node = R.node[key];
+      // byte offsets by shifts (the compiler otherwise forms key * 16 as key * 4 + key * 12, a
+      // quarter-rate multiply)
+      const uint32_t k4 = key << 2;
+      root = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(R.idx) + k4);
+      node = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(R.node) + (k4 << 2));
     } else {
-      root = w.roots[key];
-      if (root != ARC_EMPTY_CHUNK_SENTINEL) node = w.nodes[root];
+      root = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(w.roots) + (key << 2));
+      if (root != ARC_EMPTY_CHUNK_SENTINEL) node = pool_node<OFF32>(w, root);
     }
     ok = root != ARC_EMPTY_CHUNK_SENTINEL;
     if (ok) C[0] = node;
@@ -230,8 +251,9 @@ __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
   const uint32_t cx = umed3(cvt_u32(ipx), bx, bx + last);  // bx <= bx + last: boxes stay below 2^32
   const uint32_t cy = umed3(cvt_u32(ipy), by, by + last);
   const uint32_t cz = umed3(cvt_u32(ipz), bz, bz + last);
-  const uint32_t fx = (ex & L.mx) | ((bx - 1u) & ~L.mx), fy = (ey & L.my) | ((by - 1u) & ~L.my),
-                 fz = (ez & L.mz) | ((bz - 1u) & ~L.mz);
+  // first voxel beyond the box on the exit axis: ex (= b + span) going up, b - 1 (= ex - 1) going
+  // down, i.e. ex + ~m
+  const uint32_t fx = ex + ~L.mx, fy = ey + ~L.my, fz = ez + ~L.mz;
   const uint32_t nvx = is_x ? fx : cx;
   const uint32_t nvy = is_y ? fy : cy;
   const uint32_t nvz = (is_x || is_y) ? cz : fz;
