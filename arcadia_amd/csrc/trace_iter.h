@@ -39,11 +39,15 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
     // descent (:390-552): at most three loads (shift 6 -> 4 -> 2 -> 0; shift 0 always stops), as
     // four unrolled tests whose exit is wave-uniform, so the compiler keeps no per-lane loop-exit
     // masks. A lane that has stopped repeats its own test with unchanged state until the wave exits.
-    // The tests are sign bits combined in one word, so `go` is a single compare (the ballot folds
-    // into it): bit 31 of the high word of `below` = child present; bit 31 of node.z = leaf;
+    // The tests are sign bits combined in one word, so `go` is a single compare and the ballot
+    // folds into it: bit 31 of the high word of `below` = child present; bit 31 of node.z = leaf;
     // bit 31 of minsh - sh = sh > minsh (both below 8).
 #pragma unroll
     for (uint32_t pass = 0; pass < 4u; ++pass) {
+      // Opaque to the compiler: left visible, it computes the next test only inside the load
+      // block (the other lanes' state is unchanged) and then rebuilds `go` from a lane mask with
+      // two more vector instructions per pass.
+      asm volatile("" : "+v"(sh));
       b = child_bit(L, sh);
       const uint64_t below = (((uint64_t)node.y << 32) | node.x) << (63u - b);
       hw = (uint32_t)(below >> 32);
@@ -54,7 +58,8 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       }
       // Level-3 nodes are leaves in every pool the upload check accepts; sh == 0 (implied by
       // sh <= minsh) only bounds the descent if a malformed node slipped in through a patch.
-      const bool go = (int32_t)(hw & ~node.z & (minsh - sh)) < 0;
+      // (minsh - sh) & ~node.z & hw as one three-input bit operation (truth table 0x20)
+      const bool go = (int32_t)__builtin_amdgcn_bitop3_b32(minsh - sh, node.z, hw, 0x20) < 0;
       if (__builtin_amdgcn_ballot_w64(go) == 0ull) break;
       if (go) {
         node = pool_node<OFF32>(w, (node.z & ARC_PTR_INDEX_MASK) + (uint32_t)__popcll(below) - 1u);

@@ -153,6 +153,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
+    bool live = !idle_lane;  // tested again only after a refill
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     // The exit test goes through readfirstlane so the compiler sees a uniform loop exit: with an
@@ -184,11 +185,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
         if (!FULL && SEM == ARC_SEM_WORLD) s_miss[threadIdx.x] = r.t_max * 16.0f;
         ray_init<SEM>(w, p, r, L, C);
       }
+      live = fget(L.st, F_LIVE, 1) != 0u;
     }
     // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
     // `continue` back to the ballot made a second back-edge, and the register allocator then
     // copied the whole lane state on every wave trip.)
-    if (fget(L.st, F_LIVE, 1)) iteration<SEM, WATER, kMaxBlock, kLdsChunks, OFF32>(w, p, L, C, s_roots, lds_roots);
+    if (live) iteration<SEM, WATER, kMaxBlock, kLdsChunks, OFF32>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)

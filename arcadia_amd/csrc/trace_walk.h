@@ -32,7 +32,7 @@ struct Lane {
   float dx, dy, dz;  // direction used for inside points (WORLD: dir, CHUNK: safe_dir)
   float ix, iy, iz;  // 1 / safe_dir
   float nx, ny, nz;  // -origin * inv
-  uint32_t mx, my, mz;  // 0xFFFFFFFF where inv >= 0 (the box step's exit planes), else 0
+  uint32_t mx, my, mz;  // 1 where inv >= 0 (the box step exits through the upper plane), else 0
   float t, tl;       // t, t_limit
   uint32_t vx, vy, vz;
   uint32_t diff;     // prev_vi ^ vi of the coming iteration
@@ -105,6 +105,15 @@ __device__ __forceinline__ uint32_t cvt_u32(float x) { return (uint32_t)fminf(x,
 __device__ __forceinline__ uint32_t umed3(uint32_t x, uint32_t lo, uint32_t hi) {
   uint32_t r;
   asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+  return r;
+}
+
+// max(a, b) for values that are never signaling NaNs (results of arithmetic): v_max_f32 as is.
+// fmaxf also quiets its loop-carried operand first (a canonicalize, one more vector instruction
+// per box step) because the compiler cannot prove it is not a signaling NaN.
+__device__ __forceinline__ float fmax_nc(float a, float b) {
+  float r;
+  asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
   return r;
 }
 
@@ -237,27 +246,29 @@ __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4*
 // it (exit axis) or to the clamped inside point (other axes), and `diff` records what changed.
 // All three axes are computed and selected, with no branches.
 __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
-  const uint32_t mk = 0u - span;
-  const uint32_t bx = L.vx & mk, by = L.vy & mk, bz = L.vz & mk;
-  const uint32_t ex = bx + (span & L.mx), ey = by + (span & L.my), ez = bz + (span & L.mz);
+  const uint32_t last = span - 1u;
+  const uint32_t bx = L.vx & ~last, by = L.vy & ~last, bz = L.vz & ~last;
+  // exit plane: b + span going up (m = 1), b going down (m = 0); span <= 256 fits the 24-bit
+  // multiply-add
+  const uint32_t ex = __umul24(span, L.mx) + bx, ey = __umul24(span, L.my) + by, ez = __umul24(span, L.mz) + bz;
   const float tx = plane_t(ex, L.ix, L.nx), ty = plane_t(ey, L.iy, L.ny), tz = plane_t(ez, L.iz, L.nz);
   const float t_exit = fminf(fminf(tx, ty), tz);
-  const float next_t = fmaxf(L.t, t_exit);
+  const float next_t = fmax_nc(L.t, t_exit);
   const bool is_x = tx <= ty && tx <= tz;
   const bool is_y = !is_x && (ty <= tz);
   const float qx = L.dx * next_t, qy = L.dy * next_t, qz = L.dz * next_t;
   const float ipx = fmaxf(L.ox + qx, 0.0f), ipy = fmaxf(L.oy + qy, 0.0f), ipz = fmaxf(L.oz + qz, 0.0f);
-  const uint32_t last = span - 1u;
   const uint32_t cx = umed3(cvt_u32(ipx), bx, bx + last);  // bx <= bx + last: boxes stay below 2^32
   const uint32_t cy = umed3(cvt_u32(ipy), by, by + last);
   const uint32_t cz = umed3(cvt_u32(ipz), bz, bz + last);
   // first voxel beyond the box on the exit axis: ex (= b + span) going up, b - 1 (= ex - 1) going
-  // down, i.e. ex + ~m
-  const uint32_t fx = ex + ~L.mx, fy = ey + ~L.my, fz = ez + ~L.mz;
+  // down, i.e. ex + m - 1
+  const uint32_t fx = ex + L.mx - 1u, fy = ey + L.my - 1u, fz = ez + L.mz - 1u;
   const uint32_t nvx = is_x ? fx : cx;
   const uint32_t nvy = is_y ? fy : cy;
   const uint32_t nvz = (is_x || is_y) ? cz : fz;
-  L.diff = (nvx ^ L.vx) | (nvy ^ L.vy) | (nvz ^ L.vz);
+  // (a ^ b) | c as one three-input bit operation (truth table 0xBE)
+  L.diff = __builtin_amdgcn_bitop3_b32(nvz, L.vz, __builtin_amdgcn_bitop3_b32(nvy, L.vy, nvx ^ L.vx, 0xBE), 0xBE);
   L.vx = nvx;
   L.vy = nvy;
   L.vz = nvz;
@@ -355,9 +366,9 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
   L.nx = (-L.ox) * L.ix;
   L.ny = (-L.oy) * L.iy;
   L.nz = (-L.oz) * L.iz;
-  L.mx = L.ix >= 0.0f ? 0xFFFFFFFFu : 0u;
-  L.my = L.iy >= 0.0f ? 0xFFFFFFFFu : 0u;
-  L.mz = L.iz >= 0.0f ? 0xFFFFFFFFu : 0u;
+  L.mx = L.ix >= 0.0f ? 1u : 0u;
+  L.my = L.iy >= 0.0f ? 1u : 0u;
+  L.mz = L.iz >= 0.0f ? 1u : 0u;
   L.diff = 0xFFFFFFFFu;  // prev_vi starts at u32::MAX
   if (iteration_guards_fail<SEM>(w, p, L)) return;
   L.st = fput(fput(L.st, F_CSHIFT, 3, 6u), F_LIVE, 1, 1u);
