@@ -30,7 +30,7 @@ __device__ __forceinline__ void shadow_iteration(const DevWorld& w, const DevPar
                                                  const RootLds& R, bool lds_roots) {
   uint32_t st = L.st;
   uint4 node = L.node;
-  uint32_t sh = fget(st, F_CSHIFT, 3);
+  uint32_t sh = L.cs;
   const uint32_t sidx = L.step;
   L.step = sidx + 1u;
   uint32_t minsh;
@@ -83,7 +83,7 @@ __device__ __forceinline__ void shadow_iteration(const DevWorld& w, const DevPar
   }
   // one way out, as in the traversal's iteration
   L.node = node;
-  st = fput(st, F_CSHIFT, 3, sh);
+  L.cs = sh;
   if (hit) {
     st = fput(fput(st, F_LIVE, 1, 0u), F_HIT, 1, 1u);
   } else {

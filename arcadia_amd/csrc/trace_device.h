@@ -41,8 +41,12 @@ struct DevParams {
   float lod_t;           // distance threshold (voxels) for the lod_bias class
   float lod_t_far;       // lod_bias 1: sec_lod_t * 2.2857
   float ao_max_dist;     // point_light_count.w as f32
-  uint32_t pad;
+  uint32_t flags;        // kParamLodOff
 };
+
+// DevParams::flags: the LOD floor is 0 at every step (distance thresholds +inf or NaN, step
+// threshold 2^32 - 1), so the traversal skips its LOD tests.
+constexpr uint32_t kParamLodOff = 1u;
 
 __device__ __forceinline__ uint32_t f2u(float x) {
   if (!(x > 0.0f)) return 0u;

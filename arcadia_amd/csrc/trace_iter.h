@@ -17,7 +17,8 @@ namespace arcdev {
 //     popcount(mask & ((1 << b) - 1));
 //   * the descent loop only tests and loads; the material of a hit candidate (a set bit in a
 //     leaf or at the LOD floor) is resolved after the loop;
-//   * cached_shift always equals the level the iteration ended on, so one field carries both;
+//   * cached_shift always equals the level the iteration ended on, so one field (L.cs) carries
+//     both;
 //   * the box step computes the clamped inside coordinate of all three axes and selects, with no
 //     branches (box_step);
 //   * WATER = false compiles the ignore_water tests out (ray classes that never set it).
@@ -26,7 +27,7 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
                                           const RootLdsT<NCH>& R, bool lds_roots) {
   uint32_t st = L.st;
   uint4 node = L.node;
-  uint32_t sh = fget(st, F_CSHIFT, 3);
+  uint32_t sh = L.cs;
   const uint32_t sidx = L.step;
   L.step = sidx + 1u;
   const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
@@ -84,7 +85,7 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   // One way out of the iteration (a hit returns no earlier), so the lane state has one
   // definition per field where the iteration ends.
   L.node = node;
-  st = fput(st, F_CSHIFT, 3, sh);
+  L.cs = sh;
   if (hit) {
     st = fput(fput(fput(st, F_SHIFT, 3, sh), F_LIVE, 1, 0u), F_HIT, 1, 1u);
   } else {

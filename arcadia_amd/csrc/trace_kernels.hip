@@ -216,7 +216,7 @@ __device__ __forceinline__ DevParams sel_params(bool b, const DevParams& pb, con
   p.lod_t = b ? pb.lod_t : p0.lod_t;
   p.lod_t_far = b ? pb.lod_t_far : p0.lod_t_far;
   p.ao_max_dist = b ? pb.ao_max_dist : p0.ao_max_dist;
-  p.pad = 0;
+  p.flags = p0.flags & pb.flags;  // LOD tests skipped only when neither class has them (uniform)
   return p;
 }
 
@@ -381,7 +381,10 @@ static DevParams make_params(const arc_trace_params& t) {
   p.lod_t = t.lod_dist_m[cls] * 16.0f;
   p.lod_t_far = (t.lod_dist_m[1] * 16.0f) * 2.2857f;
   p.ao_max_dist = t.ao_max_dist_bits;
-  p.pad = 0;
+  // LOD never engages: t > lod_t (and t > lod_t_far for class 1) is false for +inf or NaN
+  // thresholds, sidx > lod_step is false for 2^32 - 1
+  const bool far_off = !(p.lod_t < INFINITY) && (cls != 1u || !(p.lod_t_far < INFINITY));
+  p.flags = (far_off && p.lod_step == 0xFFFFFFFFu) ? kParamLodOff : 0u;
   return p;
 }
 

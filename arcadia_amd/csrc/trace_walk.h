@@ -13,7 +13,6 @@ namespace arcdev {
 
 // packed per-lane flags
 enum : uint32_t {
-  F_CSHIFT = 0,  // 3 bits: cached_shift of the reference (level the last descent ended on)
   F_SHIFT = 3,   // 3 bits: level of the hit cell (valid when done with a hit)
   F_LIVE = 6,    // 1 bit: ray in flight
   F_HIT = 7,     // 1 bit: finished with a hit
@@ -38,6 +37,7 @@ struct Lane {
   uint32_t diff;     // prev_vi ^ vi of the coming iteration
   uint32_t step;     // outer iterations begun
   uint32_t st;       // packed flags
+  uint32_t cs;       // cached_shift of the reference: the level the last descent ended on
   uint32_t mat;      // material of the hit
   uint4 node;        // node of the level the last descent ended on
   uint32_t idx;
@@ -152,7 +152,7 @@ __device__ __forceinline__ float plane_t(uint32_t c, float inv, float noi) {
 // or lod_step and never 4. Two vector compares per step for classes 0 and 2 instead of four.
 template <uint32_t SEM>
 __device__ __forceinline__ uint32_t lod_min_shift(const DevParams& p, float t, uint32_t sidx) {
-  if (SEM == ARC_SEM_CHUNK) return 0u;
+  if (SEM == ARC_SEM_CHUNK || (p.flags & kParamLodOff)) return 0u;
   if (p.lod_bias == 1u) return (t > p.lod_t_far || sidx > p.lod_step) ? 4u : (t > p.lod_t ? 2u : 0u);
   return (t > p.lod_t || sidx > p.lod_step) ? 2u : 0u;
 }
@@ -394,7 +394,8 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
   L.mz = L.iz >= 0.0f ? 1u : 0u;
   L.diff = 0xFFFFFFFFu;  // prev_vi starts at u32::MAX
   if (iteration_guards_fail<SEM>(w, p, L)) return;
-  L.st = fput(fput(L.st, F_CSHIFT, 3, 6u), F_LIVE, 1, 1u);
+  L.cs = 6u;
+  L.st = fput(L.st, F_LIVE, 1, 1u);
 }
 
 // Device view of the pools (host side of a launch).

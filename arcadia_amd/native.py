@@ -91,14 +91,15 @@ class TraceParams(C.Structure):
 
     @classmethod
     def exact(cls, lod_bias: int = 0, ignore_water: bool = False, max_steps: int = 4096) -> "TraceParams":
-        """Full traversal: LOD off (thresholds beyond any distance/step) and a step cap no ray reaches."""
+        """Full traversal: LOD off (distance thresholds +inf, step thresholds 2^32 - 1: no distance or
+        step reaches them) and a step cap no ray reaches."""
         p = cls()
         p.semantics = ARC_SEM_WORLD
         p.lod_bias = lod_bias
         p.ignore_water = int(ignore_water)
         for i in range(3):
             p.max_steps[i] = max_steps
-            p.lod_dist_m[i] = 1.0e30
+            p.lod_dist_m[i] = float("inf")
             p.lod_step[i] = 0xFFFFFFFF
         p.ao_max_dist_bits = 24.0
         return p
