@@ -32,10 +32,11 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   L.step = sidx + 1u;
   const uint32_t minsh = lod_min_shift<SEM>(p, L.t, sidx);
   const bool descend = restart<SEM, BLOCK, NCH, OFF32>(w, L, C, R, lds_roots, minsh, sh, node);
+  const uint32_t vb = voxel_bytes(L);
   uint32_t span_mul = descend ? 1u : 4u;  // the box stepped over is span_mul << sh (empty chunk: 256)
   // Sign bit of `hw`: the voxel's child bit is set at the level the descent ended on (a hit
-  // candidate: hit unless the voxel is air/water). `b` is that child's slot.
-  uint32_t hw = 0u, b = 0u;
+  // candidate: hit unless the voxel is air/water). `s` is 63 minus that child's slot.
+  uint32_t hw = 0u, s = 0u;
   if (descend) {
     // descent (:390-552): at most three loads (shift 6 -> 4 -> 2 -> 0; shift 0 always stops), as
     // four unrolled tests whose exit is wave-uniform, so the compiler keeps no per-lane loop-exit
@@ -49,8 +50,8 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
       // block (the other lanes' state is unchanged) and then rebuilds `go` from a lane mask with
       // two more vector instructions per pass.
       asm volatile("" : "+v"(sh));
-      b = child_bit(L, sh);
-      const uint64_t below = (((uint64_t)node.y << 32) | node.x) << (63u - b);
+      s = child_shift63(vb, sh);
+      const uint64_t below = (((uint64_t)node.y << 32) | node.x) << s;
       hw = (uint32_t)(below >> 32);
       if (SEM == ARC_SEM_WORLD && WATER && p.ignore_water && (node.z & ARC_IS_UNIFORM_FLAG) != 0u &&
           (node.w >> 24) == ARC_MAT_WATER) {
@@ -73,6 +74,7 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
   const bool cand = (int32_t)hw < 0;
   bool hit = false;
   if (cand) {  // :441-538
+    const uint32_t b = s ^ 63u;
     uint32_t mat = node.w >> 24;
     if ((node.z & (ARC_IS_LEAF_FLAG | ARC_IS_UNIFORM_FLAG)) == ARC_IS_LEAF_FLAG) {
       const uint32_t word = pool_brick_word<OFF32>(w, node.z & ARC_PTR_INDEX_MASK, b >> 2);

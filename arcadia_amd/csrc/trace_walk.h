@@ -43,11 +43,12 @@ struct Lane {
   uint32_t idx;
 };
 
-// Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous:
-//   slot 0: chunk root node (after a hit: the node the hit resolved in),
-//   slot 1: cached level-1 ancestor, slot 2: cached level-2 ancestor (slot of shift s:
-//   (6 - s) / 2; the level-3 ancestor is the lane's `node`),
-//   slot 3: receives the level-3 node a descent writes like any other level (never read), so
+// Cold per-lane state in LDS, [slot][lane] so a wave's accesses are contiguous; the node cached
+// at shift s sits in slot s / 2, so its address is one shift-add of s:
+//   slot 3: chunk root node (shift 6),
+//   slot 2: cached level-1 ancestor (shift 4), slot 1: cached level-2 ancestor (shift 2); the
+//   level-3 ancestor is the lane's `node`,
+//   slot 0: receives the level-3 node a descent writes like any other level (never read), so
 //   the store needs no branch.
 constexpr uint32_t kMaxBlock = 256u;
 constexpr uint32_t kColdSlots = 4u;
@@ -179,11 +180,11 @@ __device__ __forceinline__ uint32_t restart_shift(uint32_t diff, uint32_t minsh,
   return umax_(dl, umax_(minsh, cs));
 }
 
-// Address of the LDS slot of the cached node at shift `sh` (slot (6 - sh) / 2) in a workgroup of
-// BLOCK lanes.
+// Address of the LDS slot of the cached node at shift `sh` (slot sh / 2) in a workgroup of BLOCK
+// lanes.
 template <uint32_t BLOCK = kMaxBlock>
 __device__ __forceinline__ uint4* cold_slot(uint4* C, uint32_t sh) {
-  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + (6u - sh) * (BLOCK * 8u));
+  return reinterpret_cast<uint4*>(reinterpret_cast<char*>(C) + sh * (BLOCK * 8u));
 }
 
 // Pool reads of the traversal loop. OFF32: the node pool spans at most 4 GiB (2^28 nodes) and the
@@ -259,7 +260,7 @@ __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4*
       if (root != ARC_EMPTY_CHUNK_SENTINEL) node = pool_node<OFF32>(w, root);
     }
     ok = root != ARC_EMPTY_CHUNK_SENTINEL;
-    if (ok) C[0] = node;
+    if (ok) *cold_slot<BLOCK>(C, 6u) = node;
   }
   return ok;
 }
@@ -303,6 +304,23 @@ __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
 __device__ __forceinline__ uint32_t child_bit(const Lane& L, uint32_t sh) {
   return (__builtin_amdgcn_ubfe(L.vz, sh, 2) << 4) | (__builtin_amdgcn_ubfe(L.vy, sh, 2) << 2) |
          __builtin_amdgcn_ubfe(L.vx, sh, 2);
+}
+
+// In-chunk voxel coordinates, one byte each: (x & 255) | (y & 255) << 8 | (z & 255) << 16, by two
+// byte permutes (selector 0x0C gives a zero byte).
+__device__ __forceinline__ uint32_t voxel_bytes(const Lane& L) {
+  const uint32_t xy = __builtin_amdgcn_perm(L.vy, L.vx, 0x0C0C0400u);
+  return __builtin_amdgcn_perm(L.vz, xy, 0x0C040100u);
+}
+
+// 63 - child_bit(L, sh), from vb = voxel_bytes(L): the complemented digits (3 - x, 3 - y, 3 - z),
+// one per byte, summed as (3 - x) + 4 (3 - y) + 16 (3 - z) by one 24-bit multiply with 0x1041
+// (2^12 + 2^6 + 1): the sum lands on bits 12..17 and the other partial products on bits 0-1, 6-9,
+// 20-23 and 28-29, with no carries between them. Four instructions where child_bit and the
+// subtraction take six.
+__device__ __forceinline__ uint32_t child_shift63(uint32_t vb, uint32_t sh) {
+  const uint32_t d = ~(vb >> sh) & 0x030303u;
+  return __builtin_amdgcn_ubfe(__umul24(d, 0x1041u), 12, 6);
 }
 
 // Marker a shadow walker's `mat` holds once its setup found the world entry (before that it
@@ -383,7 +401,7 @@ __device__ __forceinline__ void ray_init(const DevWorld& w, const DevParams& p, 
     L.vx = cvt_u32(clampf_(L.ox + qx, 0.0f, 256.0f - 1e-3f));
     L.vy = cvt_u32(clampf_(L.oy + qy, 0.0f, 256.0f - 1e-3f));
     L.vz = cvt_u32(clampf_(L.oz + qz, 0.0f, 256.0f - 1e-3f));
-    C[0] = rn;    // chunk root for every root restart
+    *cold_slot<kMaxBlock>(C, 6u) = rn;  // chunk root for every root restart (k_trace's workgroups)
     L.node = rn;  // `node` == cached ancestor at cached_shift (6): iteration 0 restarts here
   }
   L.nx = (-L.ox) * L.ix;
