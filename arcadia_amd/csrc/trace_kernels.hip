@@ -330,6 +330,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   bool drained = false;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
+    bool live = !idle_lane;  // tested again only after a finish / refill
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
     const bool pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1)) != 0ull;
     // drained, nothing pending, every lane idle; a uniform exit for the compiler (as in k_trace)
@@ -354,10 +355,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if (my != 0xFFFFFFFFu) path_start(w, p0, pc, my, out, L, tmax, C);
       }
+      live = fget(L.st, F_LIVE, 1) != 0u;
     }
     // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
     // straight on.
-    if (fget(L.st, F_LIVE, 1))
+    if (live)
       iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C,
                                                          s_roots, lds_roots);
   }

@@ -14,10 +14,10 @@ namespace arcdev {
 // packed per-lane flags
 enum : uint32_t {
   F_SHIFT = 3,   // 3 bits: level of the hit cell (valid when done with a hit)
-  F_LIVE = 6,    // 1 bit: ray in flight
   F_HIT = 7,     // 1 bit: finished with a hit
   F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
   F_DEPTH = 9,   // 3 bits: path kernel: 0 = primary ray, d = d-th bounce
+  F_LIVE = 31,   // 1 bit: ray in flight (the sign bit: the per-trip test is one compare)
 };
 
 __device__ __forceinline__ uint32_t fget(uint32_t st, uint32_t off, uint32_t w) { return (st >> off) & ((1u << w) - 1u); }
@@ -271,7 +271,9 @@ __device__ __forceinline__ bool restart(const DevWorld& w, const Lane& L, uint4*
 // All three axes are computed and selected, with no branches.
 __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
   const uint32_t last = span - 1u;
-  const uint32_t bx = L.vx & ~last, by = L.vy & ~last, bz = L.vz & ~last;
+  // v & ~last as one three-input bit operation (truth table 0x30), so -span is not formed
+  const uint32_t bx = __builtin_amdgcn_bitop3_b32(L.vx, last, 0u, 0x30), by = __builtin_amdgcn_bitop3_b32(L.vy, last, 0u, 0x30),
+                 bz = __builtin_amdgcn_bitop3_b32(L.vz, last, 0u, 0x30);
   // exit plane: b + span going up (m = 1), b going down (m = 0); span <= 256 fits the 24-bit
   // multiply-add
   const uint32_t ex = __umul24(span, L.mx) + bx, ey = __umul24(span, L.my) + by, ez = __umul24(span, L.mz) + bz;
