@@ -78,17 +78,21 @@ __device__ __forceinline__ LevelPos level_pos(const GridInfo& g, int l, const fl
   return lp;
 }
 
-// Trilinear interpolation of the 2 features of level l at normalized position p.
+// Trilinear interpolation of the 2 features of level l at normalized position p. Tables hold at
+// most 16 x 2^24 entries (arc_nrc_create), so byte offsets into the FP16-pair table (4 B per
+// entry) and into its FP32 gradient (8 B per entry) fit 32 bits: the loads and atomics take a
+// scalar base and a 32-bit vector offset.
 __device__ __forceinline__ void encode_level(const GridInfo& g, const uint32_t* __restrict__ table_h, int l,
                                              const float p[3], float& e0, float& e1) {
   const LevelPos lp = level_pos(g, l, p);
-  const uint32_t* t = table_h + (size_t)l * (g.table_mask + 1u);
+  const uint32_t lbase = (uint32_t)l * (g.table_mask + 1u);
   e0 = 0.0f;
   e1 = 0.0f;
   for (int k = 0; k < 8; ++k) {
     const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
     const float w = (dx ? lp.f[0] : 1.0f - lp.f[0]) * (dy ? lp.f[1] : 1.0f - lp.f[1]) * (dz ? lp.f[2] : 1.0f - lp.f[2]);
-    const uint32_t v = t[corner_index(g, l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz)];
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(table_h) +
+                                                          ((lbase + corner_index(g, l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz)) << 2));
     const float f0 = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xFFFFu));
     const float f1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
     e0 += w * f0;
@@ -100,13 +104,14 @@ __device__ __forceinline__ void scatter_level(const GridInfo& g, float* __restri
                                               float d0, float d1) {
   if (d0 == 0.0f && d1 == 0.0f) return;
   const LevelPos lp = level_pos(g, l, p);
-  float* t = grad + 2 * (size_t)l * (g.table_mask + 1u);
+  const uint32_t lbase = (uint32_t)l * (g.table_mask + 1u);
   for (int k = 0; k < 8; ++k) {
     const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
     const float w = (dx ? lp.f[0] : 1.0f - lp.f[0]) * (dy ? lp.f[1] : 1.0f - lp.f[1]) * (dz ? lp.f[2] : 1.0f - lp.f[2]);
     const uint32_t idx = corner_index(g, l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz);
-    atomicAdd(t + 2 * idx, w * d0);
-    atomicAdd(t + 2 * idx + 1, w * d1);
+    float* t = reinterpret_cast<float*>(reinterpret_cast<char*>(grad) + ((lbase + idx) << 3));
+    atomicAdd(t, w * d0);
+    atomicAdd(t + 1, w * d1);
   }
 }
 
@@ -125,7 +130,10 @@ __device__ __forceinline__ SampleIn load_sample(const arc_path_sample* s, uint32
   const arc_path_sample ps = s[i];
   si.valid = (ps.flags & ARC_PS_X1_HIT) != 0u;
   const uint32_t face = (ps.flags >> 8) & 0xFFu;
-  if (face < 6u) si.n[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
+  const float sgn = (face & 1u) ? -1.0f : 1.0f;  // selects, not a dynamic index into the array
+  si.n[0] = face < 6u && (face >> 1) == 0u ? sgn : 0.0f;
+  si.n[1] = face < 6u && (face >> 1) == 1u ? sgn : 0.0f;
+  si.n[2] = face < 6u && (face >> 1) == 2u ? sgn : 0.0f;
   for (int a = 0; a < 3; ++a) {
     const float q = (ps.pos[a] - g.bbox_min[a]) * g.inv_size;
     si.p[a] = fminf(fmaxf(q, 0.0f), 0.99999994f);  // keeps every corner inside the level grid
@@ -240,16 +248,21 @@ __global__ __launch_bounds__(256) void k_nrc_keys(GridInfo g, const arc_path_sam
 
 // perm (may be null): tile lane j handles query perm[16 * tile + j]; each query's output depends
 // on that query alone, so the visiting order changes no result.
-__global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g, const uint32_t* __restrict__ table_h,
+__global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g0, const uint32_t* __restrict__ table_h,
                                                    const _Float16* __restrict__ mlp_h,
                                                    const arc_path_sample* __restrict__ s, uint32_t n,
                                                    const uint32_t* __restrict__ perm, float4* __restrict__ out) {
   __shared__ MlpLds L;
+  // The level arrays are indexed per lane (each lane group encodes its own levels): read from this
+  // LDS copy; indexed in the kernel argument they were copied to a per-lane stack first.
+  __shared__ GridInfo sg;
+  if (threadIdx.x == 0) sg = g0;
   stage(mlp_h + kOffW1, kHid, kIn, L.w1, kPadIn, nullptr, 0);
   stage(mlp_h + kOffW2, kHid, kHid, L.w2, kPadHid, nullptr, 0);
   stage(mlp_h + kOffW3, kHid, kHid, L.w3, kPadHid, nullptr, 0);
   stage(mlp_h + kOffW4, kOut, kHid, L.w4, kPadHid, nullptr, 0);
   __syncthreads();
+  const GridInfo& g = sg;
   const uint32_t lane = threadIdx.x & 63u, grp = lane >> 4;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -267,18 +280,21 @@ __global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g, const uint32_t* _
 // Forward + backward of one batch; writes the activations and deltas ([feature][sample], FP16)
 // for the weight-gradient kernel, scatters the encoding gradient into the table gradient, and
 // accumulates the loss.
-__global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g, const uint32_t* __restrict__ table_h,
+__global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_t* __restrict__ table_h,
                                                       const _Float16* __restrict__ mlp_h,
                                                       const arc_path_sample* __restrict__ s, uint32_t n,
                                                       uint32_t ld, _Float16* __restrict__ act,
                                                       float* __restrict__ table_grad, float* __restrict__ loss) {
   __shared__ MlpLds L;
   __shared__ MlpLdsT T;
+  __shared__ GridInfo sg;  // per-lane level lookups from LDS (as in k_nrc_infer)
+  if (threadIdx.x == 0) sg = g0;
   stage(mlp_h + kOffW1, kHid, kIn, L.w1, kPadIn, T.w1t, kPadHid);
   stage(mlp_h + kOffW2, kHid, kHid, L.w2, kPadHid, T.w2t, kPadHid);
   stage(mlp_h + kOffW3, kHid, kHid, L.w3, kPadHid, T.w3t, kPadHid);
   stage(mlp_h + kOffW4, kOut, kHid, L.w4, kPadHid, T.w4t, kOut + 4);
   __syncthreads();
+  const GridInfo& g = sg;
   const uint32_t lane = threadIdx.x & 63u, grp = lane >> 4;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
