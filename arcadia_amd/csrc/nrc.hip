@@ -33,6 +33,7 @@
 namespace {
 
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kLevels = 16;
@@ -52,6 +53,16 @@ struct GridInfo {
 
 __device__ __forceinline__ floatx4 mfma(half4 a, half4 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// gfx950's double-K form: two 16-deep K tiles in one instruction. Lane group g supplies K slots
+// 8g..8g+7 of both operands; giving it the concatenation of its two 16-deep slices (a0 ++ a1 and
+// b0 ++ b1, the slices of K tiles t and t+1) pairs every weight with its activation, so the
+// product is the sum of the two 16x16x16 products.
+__device__ __forceinline__ floatx4 mfma2(half4 a0, half4 a1, half4 b0, half4 b1, floatx4 c) {
+  const half8 a = half8{a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+  const half8 b = half8{b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 __device__ __forceinline__ uint32_t corner_index(const GridInfo& g, int l, uint32_t x, uint32_t y, uint32_t z) {
@@ -201,22 +212,29 @@ __device__ __forceinline__ floatx4 forward_tile(const MlpLds& L, const half4 xb[
                                                 half4 a2[4], half4 a3[4]) {
   const floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int mt = 0; mt < 4; ++mt) {
-    floatx4 acc = z;
-    for (int kt = 0; kt < 3; ++kt) acc = mfma(lds_a(L.w1, kPadIn, mt, kt, lane), xb[kt], acc);
+    // the third 16-deep input tile is paired with a zero tile: an accumulator chained from a
+    // 32-deep into a 16-deep MFMA (either order) gave wrong sums on MI355X, so every product here
+    // is the 32-deep form
+    const half4 z4 = {0, 0, 0, 0};
+    floatx4 acc = mfma2(lds_a(L.w1, kPadIn, mt, 0, lane), lds_a(L.w1, kPadIn, mt, 1, lane), xb[0], xb[1], z);
+    acc = mfma2(lds_a(L.w1, kPadIn, mt, 2, lane), z4, xb[2], z4, acc);
     a1[mt] = relu_h(acc);
   }
   for (int mt = 0; mt < 4; ++mt) {
     floatx4 acc = z;
-    for (int kt = 0; kt < 4; ++kt) acc = mfma(lds_a(L.w2, kPadHid, mt, kt, lane), a1[kt], acc);
+    for (int kt = 0; kt < 4; kt += 2)
+      acc = mfma2(lds_a(L.w2, kPadHid, mt, kt, lane), lds_a(L.w2, kPadHid, mt, kt + 1, lane), a1[kt], a1[kt + 1], acc);
     a2[mt] = relu_h(acc);
   }
   for (int mt = 0; mt < 4; ++mt) {
     floatx4 acc = z;
-    for (int kt = 0; kt < 4; ++kt) acc = mfma(lds_a(L.w3, kPadHid, mt, kt, lane), a2[kt], acc);
+    for (int kt = 0; kt < 4; kt += 2)
+      acc = mfma2(lds_a(L.w3, kPadHid, mt, kt, lane), lds_a(L.w3, kPadHid, mt, kt + 1, lane), a2[kt], a2[kt + 1], acc);
     a3[mt] = relu_h(acc);
   }
   floatx4 y = z;
-  for (int kt = 0; kt < 4; ++kt) y = mfma(lds_a(L.w4, kPadHid, 0, kt, lane), a3[kt], y);
+  for (int kt = 0; kt < 4; kt += 2)
+    y = mfma2(lds_a(L.w4, kPadHid, 0, kt, lane), lds_a(L.w4, kPadHid, 0, kt + 1, lane), a3[kt], a3[kt + 1], y);
   return y;
 }
 
