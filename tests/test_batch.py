@@ -158,6 +158,35 @@ def test_gpu_batch_bit_exact_and_graph_replays(ctx):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("classes", [("ref", 0, "ref", 1), ("exact", 0, "ref", 2)])
+def test_gpu_batch_lod_params(ctx, classes):
+    """The path kernel with the reference-default quality settings (LOD floors and step caps on),
+    and with LOD on for the bounces only: the LOD tests then run per lane with each depth's own
+    thresholds."""
+    import torch
+
+    def params(kind, cls, water):
+        return (A.TraceParams.reference_default if kind == "ref" else A.TraceParams.exact)(cls, ignore_water=water)
+
+    pools = W.caves_world(1)
+    ctx.upload(*pools)
+    views = small_views(4, 5)
+    p0, pb = params(classes[0], classes[1], False), params(classes[2], classes[3], True)
+    ref, n = host_batch(pools, views, p0, pb, 2, 26.0, 33)
+    b = ctx.batch(views, p0, pb, n_bounces=2, bounce_max_dist_m=26.0, seed=33)
+    d = torch.zeros(b.records() * 16, dtype=torch.uint8, device="cuda")
+    b.trace(d.data_ptr())
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().view(A.HIT_DTYPE)
+    bad = W.diff_records(got, ref)
+    print(f"[batch] {classes}: {n} slots, {len(bad)} records differ", flush=True)
+    assert len(bad) == 0, f"{len(bad)} of {len(ref)} records differ, first {bad[:5]}"
+    lod = (ref["flags"] & 2) != 0  # records that stopped at an LOD floor
+    assert lod.sum() > 100, lod.sum()
+    b.close()
+
+
+@pytest.mark.gpu
 def test_gpu_batch_sparse_world_sampled_parity(ctx):
     """A larger batch on a sparse multi-chunk world: every record of a random sample of slots is
     re-derived on the host (the slot's primary ray and its bounce chain) and traced by the oracle."""
