@@ -38,13 +38,17 @@ struct arc_ctx {
   // cannot, ignore_water changes no result and traces run the kernels without the water tests.
   bool water = true;
   // persistent queues (kQueueRegionWords words each): the traversal kernel's, the path kernel's,
-  // then the shadow walkers'
+  // the shadow walkers', then two for arc_trace_bounce_device
   uint32_t* d_queue = nullptr;
   // scratch for the host-pointer entry point
   arc_ray* d_rays = nullptr;
   arc_hit* d_hits = nullptr;
   uint64_t scratch_n = 0;
   arc_launch_config cfg[3]{};  // per ray class (lod_bias 0, 1, >=2)
+  arc_launch_config cfg_bounce{};  // arc_trace_bounce_device
+  // arc_trace_bounce_device: one message word per primary ray (zero between launches)
+  uint64_t* d_msg = nullptr;
+  uint64_t msg_n = 0;
   // primary G-buffer pass: material palette and per-pass scratch
   float* d_palette = nullptr;
   arc_ray* d_gb_rays = nullptr;
@@ -88,7 +92,7 @@ int fail(arc_ctx* c, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail((ctx), ARC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
-constexpr size_t kQueueBytes = 3u * arcdev::kQueueRegionWords * sizeof(uint32_t);
+constexpr size_t kQueueBytes = 5u * arcdev::kQueueRegionWords * sizeof(uint32_t);
 
 // Traversal kernel, measured on MI355X (full-frame world, ms per primary / bounce launch, refill
 // at 16 idle lanes, windows of 64 rays): 6 workgroups of 256 per CU 0.450 / 0.429; 5: 0.464 /
@@ -114,6 +118,68 @@ bool valid_params(const arc_trace_params* p) {
 
 }  // namespace
 
+// Times every candidate launch shape with `run` (one launch on `stream`, reading the shape from
+// *slot), keeps the fastest in *slot and returns it with its median time. Candidates: workgroup
+// size, resident workgroups per CU (0 = occupancy limit), refill threshold (idle lanes before a
+// wave refills) and window size per queue claim.
+namespace {
+
+template <typename Run>
+int tune_shapes(arc_ctx* c, arc_launch_config* slot, void* stream, uint32_t reps, Run run,
+                       arc_launch_config* best, float* best_ms) {
+  if (reps == 0) reps = 3;
+  if (reps > 64) reps = 64;
+  HIP_TRY(c, hipSetDevice(c->device));
+  const arc_launch_config saved = *slot;
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  static const uint32_t kThreads[] = {128, 256};
+  static const uint32_t kBlocks[] = {0, 4, 6};
+  static const uint32_t kRefill[] = {8, 16, 24, 32, 48};
+  static const uint32_t kBatch[] = {32, 64, 128};
+  arc_launch_config win = saved;
+  float win_ms = 1e30f;
+  int rc = ARC_OK;
+  std::vector<float> t(reps);
+  for (uint32_t bt : kThreads)
+    for (uint32_t bpc : kBlocks)
+      for (uint32_t rf : kRefill)
+        for (uint32_t bat : kBatch) {
+          if (rc != ARC_OK) break;
+          *slot = arc_launch_config{bpc, bt, rf, bat};
+          rc = run();  // warm-up
+          for (uint32_t r = 0; r < reps && rc == ARC_OK; ++r) {
+            if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK) rc = run();
+            if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
+            if (rc == ARC_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
+          }
+          if (rc != ARC_OK) break;
+          std::vector<float> sorted = t;
+          std::sort(sorted.begin(), sorted.end());
+          const float med = sorted[reps / 2];
+          if (med < win_ms) {
+            win_ms = med;
+            win = *slot;
+          }
+        }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc != ARC_OK) {
+    *slot = saved;
+    return rc;
+  }
+  *slot = win;
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return ARC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int arc_create(int device, arc_ctx** out) {
@@ -125,6 +191,10 @@ int arc_create(int device, arc_ctx** out) {
   if (!c) return ARC_ERR_INVALID;
   c->device = device;
   for (auto& k : c->cfg) k = default_config();
+  c->cfg_bounce = default_config();
+  c->cfg_bounce.blocks_per_cu = 0;
+  c->cfg_bounce.refill_lanes = 24;
+  c->cfg_bounce.batch_rays = 32;
   if (hipSetDevice(device) != hipSuccess) {
     delete c;
     return ARC_ERR_HIP;
@@ -167,6 +237,7 @@ void arc_destroy(arc_ctx* c) {
   (void)hipFree(c->d_bricks);
   (void)hipFree(c->d_bodies);
   (void)hipFree(c->d_queue);
+  (void)hipFree(c->d_msg);
   (void)hipFree(c->d_rays);
   (void)hipFree(c->d_hits);
   (void)hipFree(c->d_palette);
@@ -225,13 +296,13 @@ int arc_get_launch_config(const arc_ctx* c, arc_launch_config* cfg) {
 }
 
 int arc_set_launch_config_class(arc_ctx* c, uint32_t cls, const arc_launch_config* cfg) {
-  if (!c || !cfg || cls > 2) return ARC_ERR_INVALID;
-  return normalize_config(c, cfg, &c->cfg[cls]);
+  if (!c || !cfg || cls > ARC_LAUNCH_CLASS_BOUNCE) return ARC_ERR_INVALID;
+  return normalize_config(c, cfg, cls == ARC_LAUNCH_CLASS_BOUNCE ? &c->cfg_bounce : &c->cfg[cls]);
 }
 
 int arc_get_launch_config_class(const arc_ctx* c, uint32_t cls, arc_launch_config* cfg) {
-  if (!c || !cfg || cls > 2) return ARC_ERR_INVALID;
-  *cfg = c->cfg[cls];
+  if (!c || !cfg || cls > ARC_LAUNCH_CLASS_BOUNCE) return ARC_ERR_INVALID;
+  *cfg = cls == ARC_LAUNCH_CLASS_BOUNCE ? c->cfg_bounce : c->cfg[cls];
   return ARC_OK;
 }
 
@@ -726,58 +797,78 @@ int arc_autotune(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_ra
   if (!c) return ARC_ERR_INVALID;
   if (!valid_params(params)) return fail(c, ARC_ERR_INVALID, "invalid trace params");
   if (n == 0 || !d_rays || !d_hits) return fail(c, ARC_ERR_INVALID, "autotune needs a non-empty ray set");
-  if (reps == 0) reps = 3;
-  if (reps > 64) reps = 64;
+  return tune_shapes(
+      c, &c->cfg[lod_class(params)], stream, reps, [&] { return launch(c, params, d_rays, n, d_hits, false, stream); },
+      best, best_ms);
+}
+
+// ---- primary rays and their bounces in one launch ------------------------------------------
+int arc_trace_bounce_device(arc_ctx* c, const arc_trace_params* primary, const arc_trace_params* bounce,
+                            const arc_ray* d_rays, uint64_t n, uint64_t seed, float bounce_max_dist_m,
+                            arc_hit* d_hits, void* stream) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!valid_params(primary) || !valid_params(bounce) || primary->semantics != ARC_SEM_WORLD ||
+      bounce->semantics != ARC_SEM_WORLD)
+    return fail(c, ARC_ERR_INVALID, "arc_trace_bounce_device traces world semantics (both classes)");
+  if (!c->has_world) return fail(c, ARC_ERR_NO_WORLD, "trace before arc_world_upload");
+  if (n == 0) return ARC_OK;
+  if (!d_rays || !d_hits) return fail(c, ARC_ERR_INVALID, "null ray or output pointer");
+  if (n > 0x7FFFFFFFull) return fail(c, ARC_ERR_INVALID, "at most 2^31-1 rays per launch");
   HIP_TRY(c, hipSetDevice(c->device));
-  const uint32_t cls = lod_class(params);
-  const arc_launch_config saved = c->cfg[cls];
-  hipStream_t s = (hipStream_t)stream;
-  hipEvent_t e0, e1;
-  HIP_TRY(c, hipEventCreate(&e0));
-  HIP_TRY(c, hipEventCreate(&e1));
-  // Candidate grid: workgroup size, resident workgroups per CU (0 = occupancy limit), refill
-  // threshold (idle lanes before a wave refills) and ray-window size per queue claim.
-  static const uint32_t kThreads[] = {128, 256};
-  static const uint32_t kBlocks[] = {0, 4, 6};
-  static const uint32_t kRefill[] = {8, 16, 24, 32, 48};
-  static const uint32_t kBatch[] = {32, 64, 128};
-  arc_launch_config win = saved;
-  float win_ms = 1e30f;
-  int rc = ARC_OK;
-  std::vector<float> t(reps);
-  for (uint32_t bt : kThreads)
-    for (uint32_t bpc : kBlocks)
-      for (uint32_t rf : kRefill)
-        for (uint32_t bat : kBatch) {
-          if (rc != ARC_OK) break;
-          c->cfg[cls] = arc_launch_config{bpc, bt, rf, bat};
-          rc = launch(c, params, d_rays, n, d_hits, false, stream);  // warm-up
-          for (uint32_t r = 0; r < reps && rc == ARC_OK; ++r) {
-            if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK) rc = launch(c, params, d_rays, n, d_hits, false, stream);
-            if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
-          }
-          if (rc != ARC_OK) break;
-          std::vector<float> sorted = t;
-          std::sort(sorted.begin(), sorted.end());
-          const float med = sorted[reps / 2];
-          if (med < win_ms) {
-            win_ms = med;
-            win = c->cfg[cls];
-          }
-        }
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (rc != ARC_OK) {
-    c->cfg[cls] = saved;
-    return rc;
+  if (n > c->msg_n) {
+    // the message words start at zero and every launch leaves them zero; grown (and zeroed) here
+    // only, outside any launch of the queue's stream order
+    HIP_TRY(c, hipDeviceSynchronize());
+    (void)hipFree(c->d_msg);
+    c->d_msg = nullptr;
+    c->msg_n = 0;
+    HIP_TRY(c, hipMalloc(&c->d_msg, n * sizeof(uint64_t)));
+    HIP_TRY(c, hipMemset(c->d_msg, 0, n * sizeof(uint64_t)));
+    HIP_TRY(c, hipDeviceSynchronize());
+    c->msg_n = n;
   }
-  c->cfg[cls] = win;
-  if (best) *best = win;
-  if (best_ms) *best_ms = win_ms;
+  arcdev::BounceLaunchArgs a{};
+  std::memcpy(a.grid, c->grid, sizeof a.grid);
+  a.roots = c->d_roots;
+  a.nodes = c->d_nodes;
+  a.bricks = c->d_bricks;
+  a.primary = *primary;
+  a.bounce = *bounce;
+  if (!c->water) a.primary.ignore_water = a.bounce.ignore_water = 0u;  // no water: same results without its tests
+  a.rays = d_rays;
+  a.n = n;
+  a.seed = seed;
+  a.bounce_max_dist_m = bounce_max_dist_m;
+  a.out = d_hits;
+  a.msg = c->d_msg;
+  a.off32 = pools_off32(c);
+  a.queue = c->d_queue + 3u * arcdev::kQueueRegionWords;  // two regions of its own
+  const arc_launch_config& cfg = c->cfg_bounce;
+  uint32_t bpc = cfg.blocks_per_cu;
+  if (bpc == 0) {
+    hipError_t oe = arcdev::bounce_occupancy(cfg.block_threads, &bpc);
+    if (oe != hipSuccess) return fail(c, ARC_ERR_HIP, "occupancy query: %s", hipGetErrorString(oe));
+  }
+  const uint64_t want_blocks = (uint64_t)bpc * c->n_cu;
+  const uint64_t max_useful = (n + cfg.block_threads - 1) / cfg.block_threads;
+  a.blocks = (uint32_t)std::max<uint64_t>(1u, std::min(want_blocks, max_useful));
+  a.block_threads = cfg.block_threads;
+  a.refill = cfg.refill_lanes;
+  a.batch = cfg.batch_rays;
+  hipError_t e = arcdev::launch_trace_bounce(a, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "trace+bounce launch: %s", hipGetErrorString(e));
   return ARC_OK;
+}
+
+int arc_autotune_bounce(arc_ctx* c, const arc_trace_params* primary, const arc_trace_params* bounce,
+                        const arc_ray* d_rays, uint64_t n, uint64_t seed, float bounce_max_dist_m, arc_hit* d_hits,
+                        void* stream, uint32_t reps, arc_launch_config* best, float* best_ms) {
+  if (!c) return ARC_ERR_INVALID;
+  if (n == 0 || !d_rays || !d_hits) return fail(c, ARC_ERR_INVALID, "autotune needs a non-empty ray set");
+  return tune_shapes(
+      c, &c->cfg_bounce, stream, reps,
+      [&] { return arc_trace_bounce_device(c, primary, bounce, d_rays, n, seed, bounce_max_dist_m, d_hits, stream); },
+      best, best_ms);
 }
 
 int arc_primary_gbuffer(arc_ctx* c, const arc_trace_params* params, const arc_gbuffer_params* g, uint32_t* d_depth_mat,

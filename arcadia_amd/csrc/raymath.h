@@ -100,8 +100,31 @@ ARC_HD void primary_dir(const arc_camera& cam, uint32_t px, uint32_t py, uint32_
 // make_basis frame. `ovox` = parent origin in voxels (origin_m * 16), `t_vox` the hit t in
 // voxels, `face` the compact face code (2*axis + (normal < 0)); (u1, u2) = 24-bit fractions of
 // splitmix64(seed ^ splitmix64(index + 0x51ED270B)).
-ARC_HD void bounce_ray(const float ovox[3], const float dir[3], float t_vox, uint32_t face, uint64_t seed,
-                       uint64_t index, float max_dist_m, arc_ray* out) {
+//
+// The lobe's frame depends on the face alone (nine codes, 6..8 with no normal), so it is its own
+// function: a kernel can tabulate the nine frames once and pass one to bounce_ray_framed, which
+// gives the same bits as bounce_ray.
+struct BounceFrame {
+  float sn[3], t[3], b[3];  // lobe axis normalize(n + 0.22 y) and its make_basis tangents
+};
+
+ARC_HD BounceFrame bounce_frame(uint32_t face) {
+  float nrm[3] = {0.0f, 0.0f, 0.0f};
+  if (face < 6u) nrm[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
+  const float lifted[3] = {nrm[0], nrm[1] + 0.22f, nrm[2]};
+  BounceFrame f;
+  normalize3(lifted, f.sn);
+  const bool zup = fabsf(f.sn[2]) < 0.999f;
+  const float upv[3] = {zup ? 0.0f : 1.0f, 0.0f, zup ? 1.0f : 0.0f};
+  float cr[3];
+  cross3(upv, f.sn, cr);
+  normalize3(cr, f.t);
+  cross3(f.sn, f.t, f.b);
+  return f;
+}
+
+ARC_HD void bounce_ray_framed(const float ovox[3], const float dir[3], float t_vox, uint32_t face,
+                              const BounceFrame& f, uint64_t seed, uint64_t index, float max_dist_m, arc_ray* out) {
   float nrm[3] = {0.0f, 0.0f, 0.0f};
   if (face < 6u) nrm[face >> 1] = (face & 1u) ? -1.0f : 1.0f;
   float org[3];
@@ -112,15 +135,6 @@ ARC_HD void bounce_ray(const float ovox[3], const float dir[3], float t_vox, uin
     const float off = nrm[a] * 0.10f;
     org[a] = hp + off;
   }
-  const float lifted[3] = {nrm[0], nrm[1] + 0.22f, nrm[2]};
-  float sn[3];
-  normalize3(lifted, sn);
-  const bool zup = fabsf(sn[2]) < 0.999f;
-  const float upv[3] = {zup ? 0.0f : 1.0f, 0.0f, zup ? 1.0f : 0.0f};
-  float cr[3], t[3], b[3];
-  cross3(upv, sn, cr);
-  normalize3(cr, t);
-  cross3(sn, t, b);
   const uint64_t h = splitmix64(seed ^ splitmix64(index + 0x51ED270Bull));
   const float u1 = (float)(uint32_t)(h & 0xFFFFFFu) * (1.0f / 16777216.0f);
   const float u2 = (float)(uint32_t)((h >> 24) & 0xFFFFFFu) * (1.0f / 16777216.0f);
@@ -132,7 +146,7 @@ ARC_HD void bounce_ray(const float ovox[3], const float dir[3], float t_vox, uin
   const float rc = r * cphi, rs = r * sphi;
   float d[3];
   for (int a = 0; a < 3; ++a) {
-    const float p0 = t[a] * rc, p1 = b[a] * rs, p2 = sn[a] * z;
+    const float p0 = f.t[a] * rc, p1 = f.b[a] * rs, p2 = f.sn[a] * z;
     const float s = p0 + p1;
     d[a] = s + p2;
   }
@@ -144,6 +158,11 @@ ARC_HD void bounce_ray(const float ovox[3], const float dir[3], float t_vox, uin
   out->dy = d[1];
   out->dz = d[2];
   out->t_max = max_dist_m;
+}
+
+ARC_HD void bounce_ray(const float ovox[3], const float dir[3], float t_vox, uint32_t face, uint64_t seed,
+                       uint64_t index, float max_dist_m, arc_ray* out) {
+  bounce_ray_framed(ovox, dir, t_vox, face, bounce_frame(face), seed, index, max_dist_m, out);
 }
 
 // The ray of an empty slot (padding pixel, or a bounce whose parent missed): t_max < 0 makes

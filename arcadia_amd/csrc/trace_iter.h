@@ -146,7 +146,7 @@ __device__ __forceinline__ HitInfo resolve_hit(const Lane& L, float rdx, float r
 // `miss_t`: t_max * 16 of the lane's ray (world semantics), kept in LDS from the refill so a miss
 // record needs no second global read of the ray (a register would cost 15 VGPRs here).
 template <uint32_t SEM>
-__device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays, const Lane& L, float miss_t) {
+__device__ __forceinline__ uint4 compact_record(const arc_ray* rays, const Lane& L, float miss_t) {
   uint4 v;
   const uint32_t steps = L.step > 0xFFFFu ? 0xFFFFu : L.step;
   if (fget(L.st, F_HIT, 1)) {
@@ -169,7 +169,12 @@ __device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays,
     v.z = (uint32_t)ARC_FACE_NONE << 24;
     v.w = steps;
   }
-  reinterpret_cast<uint4*>(out)[L.idx] = v;
+  return v;
+}
+
+template <uint32_t SEM>
+__device__ __forceinline__ void store_compact(arc_hit* out, const arc_ray* rays, const Lane& L, float miss_t) {
+  reinterpret_cast<uint4*>(out)[L.idx] = compact_record<SEM>(rays, L, miss_t);
 }
 
 }  // namespace arcdev

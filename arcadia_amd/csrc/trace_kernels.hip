@@ -366,6 +366,198 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
 
+// ---------------------------------------------------------------------------------------
+// Primary rays and their diffuse bounces in one launch
+// ---------------------------------------------------------------------------------------
+// A ray set and one cosine bounce off each of its hits, traced by one persistent launch. Two
+// launches (primary, then bounce) each end in a drain: once a queue runs dry, most waves idle
+// while the last long rays finish, ~0.15 ms of a 0.38 ms launch on the 1080p full-frame world.
+// Here the waves claim primary rays until that queue runs dry, then bounce indices, so the
+// primaries' drain is filled with bounce work and only the last bounces drain.
+//
+// A bounce needs its primary's hit. The lane that finishes primary i publishes one 64-bit
+// message msg[i] with a single agent-scope atomic store: 0 = not finished (the array is zero
+// between launches), kMsgMiss = no hit (the bounce slot gets the empty-ray record), else
+// (1 + face) << 32 | bits of t_voxels, which with ray i itself (read-only input) is all
+// arcmath::bounce_ray needs. The message is the whole payload, so no ordering between stores and
+// no fence is involved. Bounce index windows are claimed in index order after the primary
+// windows, so the primaries they need finished long before, except near the end; a lane whose
+// message is not there yet keeps the index (F_PEND) and polls it at later refills. The consumer
+// clears msg[i], so every launch leaves the array zero.
+//
+// Waits are bounded: a lane waits only for a primary that has been claimed, i.e. that a
+// resident wave is tracing or has finished and publishes at its next refill (once the primary
+// windows are drained, a finished primary publishes at once).
+//
+// UNI: both ray classes trace with the same parameters apart from ignore_water (LOD off in both,
+// same step cap), so the iteration reads the primary class's scalar parameters and only the water
+// flag is per lane; otherwise every field is selected per lane as in the path kernel.
+constexpr uint32_t F_PEND = 12;  // 1 bit: lane holds a bounce index whose primary has not published
+constexpr uint64_t kMsgMiss = 0xF00000000ull;
+
+__device__ __forceinline__ void msg_store(uint64_t* m, uint64_t v) {
+  __hip_atomic_store(m, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t msg_load(uint64_t* m) {
+  return __hip_atomic_load(m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounce index i's message `m` (non-zero) taken: the slot's empty record, or the bounce ray set
+// up in the lane. `frames`: arcmath::bounce_frame of the nine face codes.
+__device__ __forceinline__ void bounce_take(const DevWorld& w, const DevParams& pb, const arc_ray* rays, uint64_t m,
+                                            uint64_t seed, float bounce_tmax, uint4* out_b, Lane& L, uint4* C,
+                                            const arcmath::BounceFrame* frames) {
+  const uint32_t i = L.idx;
+  if (m == kMsgMiss) {
+    out_b[i] = empty_record();
+    L.st = 0u;
+    return;
+  }
+  const arc_ray pr = rays[i];
+  const float ov[3] = {pr.ox * 16.0f, pr.oy * 16.0f, pr.oz * 16.0f}, dr[3] = {pr.dx, pr.dy, pr.dz};
+  const uint32_t face = (uint32_t)(m >> 32) - 1u;
+  arc_ray r;
+  arcmath::bounce_ray_framed(ov, dr, __uint_as_float((uint32_t)m), face, frames[face], seed, i, bounce_tmax, &r);
+  ray_init<ARC_SEM_WORLD>(w, pb, r, L, C);
+  L.st = fput(L.st, F_DEPTH, 3, 1u);
+}
+
+// The last workgroup out resets both queues (primary windows, bounce windows).
+__device__ __forceinline__ void queue_finish2(uint32_t* queue, uint32_t* queue_b, bool leader) {
+  __syncthreads();
+  if (leader) {
+    __threadfence();
+    const uint32_t done = atomicAdd(queue + kQueueDone, 1u);
+    if (done == gridDim.x - 1u) {
+      for (uint32_t c = 0; c < kQueueCounters; ++c) {
+        atomicExch(queue + c * kQueueStride, 0u);
+        atomicExch(queue_b + c * kQueueStride, 0u);
+      }
+      atomicExch(queue + kQueueDone, 0u);
+      __threadfence();
+    }
+  }
+}
+
+// out: 2n compact records, [0, n) the primaries, [n, 2n) the bounce of primary i at n + i.
+// queue: two queue regions (primary windows, then bounce windows); msg: n words, zero.
+template <bool WATER, bool OFF32, bool UNI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_trace_bounce(
+    DevWorld w, DevParams p0, DevParams pb, const arc_ray* __restrict__ rays, uint4* __restrict__ out, uint32_t n,
+    uint64_t* __restrict__ msg, uint32_t* __restrict__ queue, uint64_t seed, float bounce_tmax, uint32_t refill,
+    uint32_t batch) {
+  __shared__ uint4 s_cold[kColdSlots * kMaxBlock];
+  __shared__ RootLds s_roots;
+  __shared__ float s_miss[kMaxBlock];  // t_max * 16 of each lane's primary ray
+  __shared__ arcmath::BounceFrame s_frame[9];  // the bounce lobe's frame per face code
+  uint4* const C = s_cold + threadIdx.x;
+  uint4* const out_b = out + n;
+  uint32_t* const queue_b = queue + kQueueRegionWords;
+  if (threadIdx.x < 9u) s_frame[threadIdx.x] = arcmath::bounce_frame(threadIdx.x);
+  __syncthreads();
+  const bool lds_roots = stage_roots(w, s_roots);
+  const uint32_t lane = __lane_id();
+  Lane L;
+  L.st = 0u;
+  L.step = 0u;
+  L.idx = 0u;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+  const float miss_b = bounce_tmax * 16.0f;
+  uint32_t wb = 0u, we = 0u;  // wave-uniform: window being handed out (primary until drained_p)
+  uint32_t qc = home_counter(), qtried = 0u, qc_b = qc, qtried_b = 0u;
+  bool drained_p = false, drained_b = false;
+  constexpr uint32_t kHold = (1u << F_HASRAY) | (1u << F_PEND);
+  // The refill block runs when nidle >= thr (wave-uniform): thr = refill while claims go on, 64
+  // once nothing is left to claim, and 0 (every trip) while a lane waits for a message or, with
+  // nothing left to claim, while the wave still traces a primary (published at once when it
+  // finishes: a bounce lane elsewhere may wait for it). In that mode the block's work (publish,
+  // claims, polling) runs when enough lanes are idle, when all are, or on every 8th entry. An
+  // ordinary trip costs one ballot and one scalar compare beyond the iteration.
+  uint32_t thr = refill, tick = 0u;
+  for (;;) {
+    const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;  // pending lanes are not live
+    bool live = !idle_lane;
+    const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
+    if (rfl((uint32_t)(nidle >= thr))) {
+      const uint64_t hold = __ballot(idle_lane && (L.st & kHold) != 0u);
+      // nothing left to claim, every lane idle and nothing held: done (uniform exit)
+      if (rfl((uint32_t)(drained_b && nidle == 64u && hold == 0ull))) break;
+      tick = rfl(tick + 1u);
+      if (rfl((uint32_t)(nidle >= refill || nidle == 64u || (tick & 7u) == 0u))) {
+        if (idle_lane && fget(L.st, F_HASRAY, 1)) {
+          const uint32_t depth = fget(L.st, F_DEPTH, 3);
+          const uint4 v = compact_record<ARC_SEM_WORLD>(rays, L, depth ? miss_b : s_miss[threadIdx.x]);
+          out[(size_t)depth * n + L.idx] = v;
+          if (depth == 0u)
+            msg_store(msg + L.idx, (v.w & ((uint32_t)ARC_HIT_FLAG_HIT << 16)) != 0u
+                                       ? ((uint64_t)(1u + (v.z >> 24)) << 32) | v.x
+                                       : kMsgMiss);
+          L.st = 0u;
+        }
+        if (!drained_b && nidle >= refill) {
+          const bool needs = idle_lane && (L.st & kHold) == 0u;
+          const uint64_t nm = __ballot(needs);
+          const uint32_t nneed = (uint32_t)__popcll(nm);
+          const uint32_t rank = (uint32_t)__popcll(nm & lt_mask);
+          uint32_t assigned = 0u, my = 0xFFFFFFFFu;
+          bool my_b = false;
+          while (assigned < nneed) {
+            if (wb >= we) {
+              if (!drained_p && !claim_window(queue, qc, qtried, batch, n, wb, we)) drained_p = true;
+              if (drained_p && !claim_window(queue_b, qc_b, qtried_b, batch, n, wb, we)) {
+                drained_b = true;
+                break;
+              }
+            }
+            const uint32_t take = umin_(nneed - assigned, we - wb);
+            if (needs && rank >= assigned && rank < assigned + take) {
+              my = wb + (rank - assigned);
+              my_b = drained_p;
+            }
+            wb += take;
+            assigned += take;
+          }
+          if (my != 0xFFFFFFFFu) {
+            L.idx = my;
+            if (my_b) {
+              L.st = 1u << F_PEND;
+            } else {
+              const arc_ray r = rays[my];
+              s_miss[threadIdx.x] = r.t_max * 16.0f;
+              ray_init<ARC_SEM_WORLD>(w, p0, r, L, C);
+            }
+          }
+        }
+        if (fget(L.st, F_PEND, 1)) {
+          const uint64_t m = msg_load(msg + L.idx);
+          if (m != 0ull) {
+            msg_store(msg + L.idx, 0ull);
+            bounce_take(w, pb, rays, m, seed, bounce_tmax, out_b, L, C, s_frame);
+          }
+        }
+      }
+      live = fget(L.st, F_LIVE, 1) != 0u;
+      const bool pend = __ballot(fget(L.st, F_PEND, 1) != 0u) != 0ull;
+      const bool prim = drained_b && __ballot((L.st & (1u << F_HASRAY)) != 0u && fget(L.st, F_DEPTH, 3) == 0u) != 0ull;
+      thr = rfl((pend || prim) ? 0u : (drained_b ? 64u : refill));
+      // nothing to trace and a message still missing: poll more slowly (~500 cycles)
+      if (rfl((uint32_t)(pend && __ballot(live) == 0ull))) __builtin_amdgcn_s_sleep(8);
+    }
+    if (live) {
+      const bool b = fget(L.st, F_DEPTH, 3) != 0u;
+      if (UNI) {
+        DevParams pl = p0;
+        if (WATER) pl.ignore_water = b ? pb.ignore_water : p0.ignore_water;
+        iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(w, pl, L, C, s_roots, lds_roots);
+      } else {
+        iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(w, sel_params(b, pb, p0), L, C, s_roots,
+                                                                      lds_roots);
+      }
+    }
+  }
+  queue_finish2(queue, queue_b, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
+}
+
 }  // namespace arcdev
 
 // ---------------------------------------------------------------------------------------
@@ -472,6 +664,44 @@ hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream) {
   else if (large) launch_path_as<kPathLargeBlock, kLdsChunksLarge, false>(a, w, pc, stream);
   else if (water) launch_path_as<kMaxBlock, kLdsChunks, true>(a, w, pc, stream);
   else launch_path_as<kMaxBlock, kLdsChunks, false>(a, w, pc, stream);
+  return hipGetLastError();
+}
+
+// The two classes trace alike apart from ignore_water: LOD never engages in either and the step
+// caps agree (lod_bias then only selects the LOD rule, which is off).
+static bool params_uniform(const DevParams& p0, const DevParams& pb) {
+  return (p0.flags & kParamLodOff) && (pb.flags & kParamLodOff) && p0.max_steps == pb.max_steps;
+}
+
+hipError_t bounce_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, reinterpret_cast<const void*>(&k_trace_bounce<true, true, false>), (int)block_threads, 0);
+  *blocks_per_cu = nb > 0 ? (uint32_t)nb : 1u;
+  return e;
+}
+
+template <bool WATER, bool OFF32>
+static void launch_bounce_as(const BounceLaunchArgs& a, const DevWorld& w, const DevParams& p0, const DevParams& pb,
+                             hipStream_t stream) {
+  uint4* const out = reinterpret_cast<uint4*>(a.out);
+  if (params_uniform(p0, pb))
+    hipLaunchKernelGGL((k_trace_bounce<WATER, OFF32, true>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p0,
+                       pb, a.rays, out, (uint32_t)a.n, a.msg, a.queue, a.seed, a.bounce_max_dist_m, a.refill, a.batch);
+  else
+    hipLaunchKernelGGL((k_trace_bounce<WATER, OFF32, false>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                       p0, pb, a.rays, out, (uint32_t)a.n, a.msg, a.queue, a.seed, a.bounce_max_dist_m, a.refill,
+                       a.batch);
+}
+
+hipError_t launch_trace_bounce(const BounceLaunchArgs& a, hipStream_t stream) {
+  const DevWorld w = make_world(a.grid, a.roots, a.nodes, a.bricks);
+  const DevParams p0 = make_params(a.primary), pb = make_params(a.bounce);
+  const bool water = a.primary.ignore_water || a.bounce.ignore_water;
+  if (water && a.off32) launch_bounce_as<true, true>(a, w, p0, pb, stream);
+  else if (water) launch_bounce_as<true, false>(a, w, p0, pb, stream);
+  else if (a.off32) launch_bounce_as<false, true>(a, w, p0, pb, stream);
+  else launch_bounce_as<false, false>(a, w, p0, pb, stream);
   return hipGetLastError();
 }
 

@@ -68,6 +68,26 @@ struct PathLaunchArgs {
   uint32_t blocks, block_threads, refill, batch;
 };
 hipError_t launch_path(const PathLaunchArgs& a, hipStream_t stream);
+
+// One launch of a ray set and one diffuse bounce off each of its hits (k_trace_bounce).
+struct BounceLaunchArgs {
+  uint32_t grid[3];
+  const uint32_t* roots;
+  const arc_node64* nodes;
+  const arc_brick64* bricks;
+  arc_trace_params primary, bounce;
+  const arc_ray* rays;
+  uint64_t n;
+  uint64_t seed;
+  float bounce_max_dist_m;
+  arc_hit* out;      // 2n records: primaries, then the bounce of primary i at n + i
+  uint64_t* msg;     // n words, zero between launches
+  bool off32;        // node and brick pools each within 4 GiB (see pool_node)
+  uint32_t* queue;   // 2 * kQueueRegionWords u32 words, zero between launches
+  uint32_t blocks, block_threads, refill, batch;
+};
+hipError_t launch_trace_bounce(const BounceLaunchArgs& a, hipStream_t stream);
+hipError_t bounce_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu);
 // Chunk roots staged in LDS (root index + root node, 20 B per chunk): kLdsChunks covers the
 // 4x4x4-chunk worlds in 5 KB per workgroup of 256 lanes; kLdsChunksLarge covers a 16x16x16-chunk
 // world in 80 KB, held by the path kernel's large shape, one workgroup of kPathLargeBlock lanes

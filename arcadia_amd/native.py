@@ -23,6 +23,7 @@ ARC_ERR_CORRUPT = -6
 
 ARC_SEM_WORLD = 0
 ARC_SEM_CHUNK = 1
+ARC_LAUNCH_CLASS_BOUNCE = 3  # arc_trace_bounce_device launch shape slot
 ARC_GEN_HEIGHTMAP = 0
 ARC_GEN_CAVES = 1
 ARC_GEN_SPARSE = 2
@@ -314,6 +315,11 @@ _SIGS = [
     ("arc_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
     ("arc_set_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
     ("arc_get_launch_config_class", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig)]),
+    ("arc_trace_bounce_device", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(TraceParams), C.c_void_p,
+                                         C.c_uint64, C.c_uint64, C.c_float, C.c_void_p, C.c_void_p]),
+    ("arc_autotune_bounce", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.POINTER(TraceParams), C.c_void_p,
+                                     C.c_uint64, C.c_uint64, C.c_float, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.POINTER(LaunchConfig), C.POINTER(C.c_float)]),
     ("arc_autotune", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                C.c_uint32, C.POINTER(LaunchConfig), C.POINTER(C.c_float)]),
     ("arc_world_host_create", C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
@@ -870,6 +876,23 @@ class Context:
         cfg, ms = LaunchConfig(), C.c_float()
         self._check(lib().arc_autotune(self._h, C.byref(params), d_rays, n, d_hits, stream or None, reps,
                                        C.byref(cfg), C.byref(ms)), "arc_autotune")
+        return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}, ms.value
+
+    def trace_bounce_device(self, primary: TraceParams, bounce: TraceParams, d_rays: int, n: int, seed: int,
+                            bounce_max_dist_m: float, d_hits: int, stream: int = 0) -> None:
+        """arc_trace_bounce_device: the n rays and one diffuse bounce off each hit in one launch; d_hits
+        receives 2n compact records (the rays', then the bounce of ray i at n + i)."""
+        self._check(lib().arc_trace_bounce_device(self._h, C.byref(primary), C.byref(bounce), d_rays, n, seed,
+                                                  bounce_max_dist_m, d_hits, stream or None),
+                    "arc_trace_bounce_device")
+
+    def autotune_bounce(self, primary: TraceParams, bounce: TraceParams, d_rays: int, n: int, seed: int,
+                        bounce_max_dist_m: float, d_hits: int, stream: int = 0, reps: int = 3):
+        """Native autotuner for arc_trace_bounce_device's launch shape on this ray set."""
+        cfg, ms = LaunchConfig(), C.c_float()
+        self._check(lib().arc_autotune_bounce(self._h, C.byref(primary), C.byref(bounce), d_rays, n, seed,
+                                              bounce_max_dist_m, d_hits, stream or None, reps, C.byref(cfg),
+                                              C.byref(ms)), "arc_autotune_bounce")
         return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}, ms.value
 
     def trace(self, params: TraceParams, rays: np.ndarray) -> np.ndarray:

@@ -224,6 +224,19 @@ int arc_trace_pools_noalloc(const uint32_t grid[3], const uint32_t* d_roots, con
 /* Full TraceResult records (hit position, normal, lod rgb, voxel hash, contact AO). */
 int arc_trace_full_device(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays,
                           uint64_t n, arc_trace_result* d_out, void* stream);
+/* A ray set and one cosine-weighted diffuse bounce off each of its hits, in ONE persistent launch.
+ * Replaces the primary trace followed by the GI pass's bounce trace (the reference dispatches
+ * primary_raymarch, then traces the bounce of lighting_restir_gi.wgsl:195-209 in its own pass).
+ * The bounce of ray i is arc_gen_bounce_rays_dense's ray for index i (bounce_ray with the hit's
+ * t_voxels and face, `seed`, hash index i, t_max = bounce_max_dist_m), generated on the device
+ * once ray i has finished. d_hits receives 2n compact records: [0, n) the rays' records, equal
+ * to arc_trace_device with `primary`; n + i the record of ray i's bounce traced with `bounce`,
+ * or the empty-ray record (t_voxels -16, no hit, 0 steps) where ray i missed. Both classes use
+ * world semantics. The context keeps one 8 B word per ray (grown on the first larger call).
+ * Launches on one context must not overlap (one queue). */
+int arc_trace_bounce_device(arc_ctx* ctx, const arc_trace_params* primary, const arc_trace_params* bounce,
+                            const arc_ray* d_rays, uint64_t n, uint64_t seed, float bounce_max_dist_m,
+                            arc_hit* d_hits, void* stream);
 
 /* ---- shadow any-hit walkers ------------------------------------------------------------- */
 /* trace_shadow_vis (shaders/common/traversal64.wgsl:571-773): sun / point-light visibility,
@@ -462,7 +475,9 @@ typedef struct arc_launch_config {
 /* Set the launch shape of every ray class (lod_bias 0, 1, >=2) / read the class-0 shape. */
 int arc_set_launch_config(arc_ctx* ctx, const arc_launch_config* cfg);
 int arc_get_launch_config(const arc_ctx* ctx, arc_launch_config* cfg);
-/* Launch shape of one ray class (lod_class 0, 1, 2 = the lod_bias classes of the params). */
+/* Launch shape of one ray class (lod_class 0, 1, 2 = the lod_bias classes of the params), or of
+ * arc_trace_bounce_device's launch (ARC_LAUNCH_CLASS_BOUNCE). */
+#define ARC_LAUNCH_CLASS_BOUNCE 3u
 int arc_set_launch_config_class(arc_ctx* ctx, uint32_t lod_class, const arc_launch_config* cfg);
 int arc_get_launch_config_class(const arc_ctx* ctx, uint32_t lod_class, arc_launch_config* cfg);
 /* Autotuner (no reference counterpart; the reference dispatches fixed 8x8 workgroups): times
@@ -472,6 +487,11 @@ int arc_get_launch_config_class(const arc_ctx* ctx, uint32_t lod_class, arc_laun
 int arc_autotune(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n,
                  arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
                  float* best_ms);
+/* The same search for arc_trace_bounce_device's launch on this ray set (d_hits: 2n records). */
+int arc_autotune_bounce(arc_ctx* ctx, const arc_trace_params* primary, const arc_trace_params* bounce,
+                        const arc_ray* d_rays, uint64_t n, uint64_t seed, float bounce_max_dist_m,
+                        arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
+                        float* best_ms);
 
 /* ---- batched views (one persistent launch) ----------------------------------------------- */
 /* A batch of camera views of uneven sizes traced with their diffuse bounces in ONE persistent
