@@ -5,7 +5,10 @@ Workload (one "step"): the full-frame traversal of a seeded 1024^3 world (4x4x4 
 cuboids of edge 2..64 voxels, random materials 1..15): 1920x1080 primary rays from a pinhole
 camera (lod_bias 0, 250 m), then one cosine-weighted diffuse bounce ray per hit pixel
 (lod_bias 2, ignore_water, 26 m). Exact traversal (no LOD cut-off, no step truncation).
-Rays are resident in HBM before timing; a step is two launches of the traversal kernel.
+The primary rays are resident in HBM before timing; a step is one launch of
+arc_trace_bounce_device, which traces them and generates and traces each hit's bounce on the
+device. The same rays traced as two launches (primary, then the host-generated bounce rays) are
+timed beside it under config.split.
 
 Prints one JSON line (metric = Grays/s over primary+diffuse rays). A second workload is reported
 beside it under "batched": 64 camera views of uneven sizes (width log-uniform in 320..2560,
@@ -804,6 +807,7 @@ def main() -> int:
     ap.add_argument("--launch-primary", default="", help="fixed launch shape of the primary-ray launches, "
                     "blocks_per_cu,block_threads,refill_lanes,batch_rays (no autotuning of that class)")
     ap.add_argument("--launch-bounce", default="", help="the same for the bounce-ray launches")
+    ap.add_argument("--launch-fused", default="", help="the same for the one-launch primary + bounce step")
     ap.add_argument("--no-c1", action="store_true", help="skip the single-chunk raycast_chunk_tree workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-views workload")
     ap.add_argument("--no-shadow", action="store_true", help="skip the shadow-walker workload")
@@ -853,15 +857,18 @@ def main() -> int:
     ctx.trace_device(p_primary, d_prays.data_ptr(), len(prays), d_phits.data_ptr(), sptr)
     torch.cuda.synchronize()
     phits = d_phits.cpu().numpy().view(A.HIT_DTYPE)
-    brays, _ = A.bounce_rays(prays, phits, seed=args.seed, max_dist_m=26.0)
+    brays, bsrc = A.bounce_rays(prays, phits, seed=args.seed, max_dist_m=26.0)
     d_brays = torch.from_numpy(brays.view(np.uint8)).cuda()
     d_bhits = torch.zeros(max(len(brays), 1) * 16, dtype=torch.uint8, device="cuda")
     n_p, n_b = len(prays), len(brays)
     print(f"[bench] {n_p} primary rays, {n_b} bounce rays", file=sys.stderr, flush=True)
 
+    d_fhits = torch.zeros(2 * n_p * 16, dtype=torch.uint8, device="cuda")
+    bounce_seed, bounce_m = args.seed, 26.0
+
     tuned = {}
     fixed = {}
-    for cls, spec in ((0, args.launch_primary), (2, args.launch_bounce)):
+    for cls, spec in ((0, args.launch_primary), (2, args.launch_bounce), (A.ARC_LAUNCH_CLASS_BOUNCE, args.launch_fused)):
         if spec:
             bpc, bt, rf, bat = (int(x) for x in spec.split(","))
             ctx.set_launch_config_class(cls, blocks_per_cu=bpc, block_threads=bt, refill_lanes=rf, batch_rays=bat)
@@ -874,8 +881,21 @@ def main() -> int:
             cfg, ms = ctx.autotune(params, d_r.data_ptr(), n_r, d_h.data_ptr(), sptr)
             tuned[label] = cfg
             print(f"[bench] autotuned {label}: {cfg} ({ms:.3f} ms)", file=sys.stderr, flush=True)
+        if A.ARC_LAUNCH_CLASS_BOUNCE not in fixed:
+            cfg, ms = ctx.autotune_bounce(p_primary, p_bounce, d_prays.data_ptr(), n_p, bounce_seed, bounce_m,
+                                          d_fhits.data_ptr(), sptr)
+            tuned["fused"] = cfg
+            print(f"[bench] autotuned fused: {cfg} ({ms:.3f} ms)", file=sys.stderr, flush=True)
 
-    def step(evs=None):
+    def step(ev0=None, ev1=None):
+        if ev0 is not None:
+            ev0.record(stream)
+        ctx.trace_bounce_device(p_primary, p_bounce, d_prays.data_ptr(), n_p, bounce_seed, bounce_m,
+                                d_fhits.data_ptr(), sptr)
+        if ev1 is not None:
+            ev1.record(stream)
+
+    def split_step(evs=None):
         if evs is not None:
             evs[0].record(stream)
         ctx.trace_device(p_primary, d_prays.data_ptr(), n_p, d_phits.data_ptr(), sptr)
@@ -894,22 +914,32 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(args.steps)]
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     start.record(stream)
     for k in range(args.steps):
-        step(evs[k])
+        step(*evs[k])
     end.record(stream)
     torch.cuda.synchronize()
     total_ms = start.elapsed_time(end)
-    ms_p = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    ms_b = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    ms_launch = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     ms_step = total_ms / args.steps
     rays_per_step = n_p + n_b
     value = rays_per_step / (ms_step * 1e-3) / 1e9
-    print(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step (primary {ms_p:.3f} ms, bounce {ms_b:.3f} ms) "
-          f"-> {value:.3f} Grays/s", file=sys.stderr, flush=True)
+    # the same rays as two launches (bounce rays generated on the host beforehand), for comparison
+    for _ in range(args.warmup):
+        split_step()
+    torch.cuda.synchronize()
+    sevs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    for k in range(args.steps):
+        split_step(sevs[k])
+    torch.cuda.synchronize()
+    ms_p = float(np.mean([e[0].elapsed_time(e[1]) for e in sevs]))
+    ms_b = float(np.mean([e[1].elapsed_time(e[2]) for e in sevs]))
+    print(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step (one launch {ms_launch:.3f} ms) -> {value:.3f} Grays/s; "
+          f"two launches: primary {ms_p:.3f} ms + bounce {ms_b:.3f} ms -> {rays_per_step / ((ms_p + ms_b) * 1e-3) / 1e9:.3f} "
+          f"Grays/s", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -926,26 +956,34 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "full-frame traversal: 1024^3 world (4x4x4 chunks, 3D fBm caves @0.45 + 2000 cuboids), "
-                        "1920x1080 primary + 1 cosine diffuse bounce per hit, exact (no LOD, no step cap)",
+                        "1920x1080 primary + 1 cosine diffuse bounce per hit, exact (no LOD, no step cap); "
+                        "one launch per step (arc_trace_bounce_device: bounces generated on the device)",
             "world_nodes": int(len(pools[2])),
             "world_bricks": int(len(pools[3])),
             "primary_rays": int(n_p),
             "bounce_rays": int(n_b),
+            "ms_launch": round(ms_launch, 4),
+            "launch": ctx.launch_config(A.ARC_LAUNCH_CLASS_BOUNCE),
             "primary_grays": round(n_p / (ms_p * 1e-3) / 1e9, 4),
             "bounce_grays": round(n_b / (ms_b * 1e-3) / 1e9, 4),
-            "ms_primary": round(ms_p, 4),
-            "ms_bounce": round(ms_b, 4),
-            "launch": {"primary": ctx.launch_config(0), "bounce": ctx.launch_config(2),
-                       "autotuned": sorted(tuned)},
+            "split": {
+                "note": "the same rays as two launches of k_trace (primary, then the bounce rays generated on "
+                        "the host before timing)",
+                "grays": round(rays_per_step / ((ms_p + ms_b) * 1e-3) / 1e9, 4),
+                "ms_primary": round(ms_p, 4),
+                "ms_bounce": round(ms_b, 4),
+                "launch": {"primary": ctx.launch_config(0), "bounce": ctx.launch_config(2)},
+            },
+            "autotuned": sorted(tuned),
             "parallelism": "single GPU",
         },
     }
     if not args.no_cpu_baseline:
         cpu, bytes_per = cpu_baseline(pools, prays, brays, p_primary, p_bounce, threads, cpu_info,
                                       args.cpu_budget)
-        # Two instantiations of the traversal kernel run per step: k_trace<WORLD, compact, water off>
-        # on the primary rays and k_trace<WORLD, compact, water on> on the bounce rays. The roofline
-        # is the primary-ray kernel's; the bounce kernel's figures sit beside it.
+        # One kernel runs per step: k_trace_bounce on the primary rays and their bounces. Its
+        # algorithmic bytes are the primary rays' and the bounce rays' (the oracle's per-ray load
+        # counts); the two k_trace launches of the same rays sit beside it under "split".
         def roof(kind, n, ms):
             achieved = bytes_per[kind] * n / (ms * 1e-3) / 1e9
             return {"achieved": round(achieved, 2), "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -953,40 +991,59 @@ def main() -> int:
                     "avg_launch_ms": round(ms, 4)}
         rp, rb = roof("primary", n_p, ms_p), roof("bounce", n_b, ms_b)
         ev = measured_counters()
+        bytes_launch = bytes_per["primary"] * n_p + bytes_per["bounce"] * n_b
+        achieved = bytes_launch / (ms_launch * 1e-3) / 1e9
+        traffic = measured_traffic("fused")
         out["roofline"] = {
             # what bounds the kernel, from its rocprofv3 counters: VALU busy near saturation with HBM
             # traffic a few percent of peak is issue-bound; `achieved` / `frac` stay the algorithmic
             # (logical) bytes per launch over HBM peak
-            "bound": ev.get("bound", "unmeasured"),
-            "achieved": rp["achieved"],
+            "bound": ev.get("fused_bound", "unmeasured"),
+            "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": rp["frac"],
-            "traffic": rp["traffic"],
-            "hbm_gbs": None if rp["traffic"] is None else round(rp["traffic"] / (ms_p * 1e-3) / 1e9, 1),
-            "l2_hit": ev.get("primary_l2_hit"),
-            "valu_busy": ev.get("primary_valu_busy"),
-            "lanes_active": ev.get("primary_lanes_active"),
-            "kernel": "k_trace<WORLD, compact, ignore_water off> on the primary rays",
-            "bytes_per_ray": rp["bytes_per_ray"],
-            "avg_launch_ms": rp["avg_launch_ms"],
-            "steps_per_ray": round(bytes_per["steps_primary"], 2),
-            "bounce": dict(rb, kernel="k_trace<WORLD, compact, ignore_water on> on the bounce rays",
-                           steps_per_ray=round(bytes_per["steps_bounce"], 2),
-                           hbm_gbs=None if rb["traffic"] is None else round(rb["traffic"] / (ms_b * 1e-3) / 1e9, 1),
-                           l2_hit=ev.get("bounce_l2_hit"), valu_busy=ev.get("bounce_valu_busy"),
-                           lanes_active=ev.get("bounce_lanes_active")),
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "hbm_gbs": None if traffic is None else round(traffic / (ms_launch * 1e-3) / 1e9, 1),
+            "l2_hit": ev.get("fused_l2_hit"),
+            "valu_busy": ev.get("fused_valu_busy"),
+            "lanes_active": ev.get("fused_lanes_active"),
+            "kernel": "k_trace_bounce<water off, off32, uniform params>: primary rays and their on-device bounces",
+            "bytes_per_launch": int(bytes_launch),
+            "bytes_per_ray": {"primary": rp["bytes_per_ray"], "bounce": rb["bytes_per_ray"]},
+            "steps_per_ray": {"primary": round(bytes_per["steps_primary"], 2),
+                              "bounce": round(bytes_per["steps_bounce"], 2)},
+            "avg_launch_ms": round(ms_launch, 4),
+            "split": {
+                "primary": dict(rp, kernel="k_trace<WORLD, compact, ignore_water off> on the primary rays",
+                                hbm_gbs=None if rp["traffic"] is None else round(rp["traffic"] / (ms_p * 1e-3) / 1e9, 1),
+                                l2_hit=ev.get("primary_l2_hit"), valu_busy=ev.get("primary_valu_busy"),
+                                lanes_active=ev.get("primary_lanes_active")),
+                "bounce": dict(rb, kernel="k_trace<WORLD, compact, ignore_water on> on the bounce rays",
+                               hbm_gbs=None if rb["traffic"] is None else round(rb["traffic"] / (ms_b * 1e-3) / 1e9, 1),
+                               l2_hit=ev.get("bounce_l2_hit"), valu_busy=ev.get("bounce_valu_busy"),
+                               lanes_active=ev.get("bounce_lanes_active")),
+            },
             "evidence": ev.get("source"),
         }
         out["cpu_baseline"] = cpu
-        # the records of the last timed launches against the oracle on the strided samples
+        # the records of the last timed launches against the oracle on the strided samples: the
+        # one-launch step's (primary i at i, its bounce at n + i) and the two launches'
+        got_f = d_fhits.cpu().numpy().view(A.HIT_DTYPE)
         got_p = d_phits.cpu().numpy().view(A.HIT_DTYPE)
         got_b = d_bhits.cpu().numpy().view(A.HIT_DTYPE)[:n_b]
         exact = {}
-        for kind, got in (("primary", got_p), ("bounce", got_b)):
-            idx, ref = bytes_per["ref_" + kind]
+        idx_p, ref_p = bytes_per["ref_primary"]
+        idx_b, ref_b = bytes_per["ref_bounce"]
+        for kind, got, idx, ref in (("primary", got_f[:n_p], idx_p, ref_p), ("bounce", got_f[n_p:][bsrc], idx_b, ref_b),
+                                    ("split_primary", got_p, idx_p, ref_p), ("split_bounce", got_b, idx_b, ref_b)):
             exact[kind] = bool(np.array_equal(got[idx].view(np.uint32), ref.view(np.uint32)))
             exact[kind + "_rays"] = int(len(idx))
+        miss = np.ones(n_p, bool)
+        miss[bsrc] = False
+        empty = got_f[n_p:][miss]
+        exact["bounce_of_misses_empty"] = bool(np.all(empty["t_voxels"] == -16.0) and np.all(empty["steps"] == 0)
+                                               and np.all((empty["flags"] & 1) == 0))
         out["oracle_sample_exact"] = exact
         print(f"[bench] timed records vs oracle: {exact}", file=sys.stderr, flush=True)
     if not args.no_c1:
@@ -1002,7 +1059,7 @@ def main() -> int:
     if not args.no_restir:
         out["restir"] = restir_leg(args, ctx, pools, prays, phits, stream, threads)
     ctx.close()
-    del d_prays, d_phits, d_brays, d_bhits
+    del d_prays, d_phits, d_brays, d_bhits, d_fhits
     if not args.no_batched:
         out["batched"] = batched_leg(args, threads, stream)
         print(f"[bench] batched: {out['batched']['value']} Grays/s, {out['batched']['ms_per_frame']} ms/frame",

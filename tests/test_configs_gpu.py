@@ -1,8 +1,9 @@
 """The five benchmark configurations at their stated sizes on the GPU, each checked against the oracle.
 
   * full frame: 1024^3 cave world (4x4x4 chunks, 3D fBm at 0.45 + 2000 cuboids), 1920x1080
-    primary rays and one cosine bounce per hit; strided samples of both launches compared with the
-    oracle record for record (every 8th primary ray, every 8th bounce ray);
+    primary rays and one cosine bounce per hit, traced as two launches and as one
+    (arc_trace_bounce_device); strided samples compared with the oracle record for record (every
+    8th primary ray, every 8th bounce ray);
   * batched views: 64 views of uneven sizes (width log-uniform in 320..2560) over the 4096^3 sparse
     world, primary + 2 bounces in one persistent launch, replayed from its hipGraph; 20,000 random
     slots re-derived on the host and traced by the oracle at every depth;
@@ -60,20 +61,33 @@ def test_full_frame_primary_and_bounce(ctx):
     ctx.trace_device(p0, d_r.data_ptr(), len(prays), d_h.data_ptr())
     torch.cuda.synchronize()
     ph = d_h.cpu().numpy().view(A.HIT_DTYPE)
-    brays, _ = A.bounce_rays(prays, ph, seed=1, max_dist_m=26.0)
+    brays, bsrc = A.bounce_rays(prays, ph, seed=1, max_dist_m=26.0)
     assert len(brays) > 1_000_000
     d_b = torch.from_numpy(brays.view(np.uint8)).cuda()
     d_bh = torch.zeros(len(brays) * 16, dtype=torch.uint8, device="cuda")
     ctx.trace_device(pb, d_b.data_ptr(), len(brays), d_bh.data_ptr())
     torch.cuda.synchronize()
     bh = d_bh.cpu().numpy().view(A.HIT_DTYPE)
+    # the same frame as one launch: primaries, then each hit's bounce generated on the device
+    n = len(prays)
+    d_f = torch.zeros(2 * n * 16, dtype=torch.uint8, device="cuda")
+    ctx.trace_bounce_device(p0, pb, d_r.data_ptr(), n, 1, 26.0, d_f.data_ptr())
+    torch.cuda.synchronize()
+    fh = d_f.cpu().numpy().view(A.HIT_DTYPE)
+    miss = np.ones(n, bool)
+    miss[bsrc] = False
+    assert np.all(fh[n:][miss]["t_voxels"] == -16.0) and np.all(fh[n:][miss]["steps"] == 0)
     ow = O.World(*pools)
-    for label, params, rays, got in (("primary", p0, prays, ph), ("bounce", pb, brays, bh)):
+    for label, params, rays, got, fused in (("primary", p0, prays, ph, fh[:n]), ("bounce", pb, brays, bh, fh[n:][bsrc])):
         idx = np.arange(0, len(rays), 8)
         full, _ = O.trace_world(ow, params, rays[idx], THREADS)
-        bad = W.diff_records(got[idx], O.compact(full))
-        print(f"[config] full frame {label}: {len(idx)} sampled records, {len(bad)} differ", flush=True)
+        ref = O.compact(full)
+        bad = W.diff_records(got[idx], ref)
+        bad_f = W.diff_records(fused[idx], ref)
+        print(f"[config] full frame {label}: {len(idx)} sampled records, {len(bad)} differ (two launches), "
+              f"{len(bad_f)} differ (one launch)", flush=True)
         assert len(bad) == 0, f"{label}: first differing sample indices {idx[bad[:5]]}"
+        assert len(bad_f) == 0, f"{label}, one launch: first differing sample indices {idx[bad_f[:5]]}"
 
 
 def batched_views(seed=1):
