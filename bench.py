@@ -627,7 +627,7 @@ def cache_leg(args, threads):
     ctx.upload(*pools)
     cam = A.camera_look((31.3, 58.0, 27.1), 0.7, -0.5, 65.0, 1920, 1080)
     run = LC.CacheRun(ctx, cam, 1920, 1080, train_size=(344, 192), sharc_log2=20,
-                      nrc=A.NrcParams.default(bbox_size=64.0, log2_table=19), seed=args.seed)
+                      nrc=A.NrcParams.default(bbox_size=64.0, log2_table=14), seed=args.seed)
     print(f"[bench] cache: scene {time.time() - t0:.1f}s, {run.n_eval} eval slots, {run.n_train} training paths "
           f"per frame", file=sys.stderr, flush=True)
     ref_s = run.reference(args.cache_ref_spp, progress=lambda k: print(f"[bench] cache: reference {k} spp",
@@ -672,7 +672,7 @@ def cache_leg(args, threads):
         "ref_seconds": round(ref_s, 2),
         "scene": "full-frame world + 8 emissive clusters (mats 19-22, r 6..14 voxels), analytic sky, 4-bounce paths",
         "sharc_entries": 1 << 22,
-        "nrc_model": {"hash_levels": 16, "table": 1 << 19, "mlp": "38->64->64->64->3 FP16 MFMA, FP32 accumulate"},
+        "nrc_model": {"hash_levels": 16, "table": 1 << 14, "mlp": "38->64->64->64->3 FP16 MFMA, FP32 accumulate"},
     }
     run.close()
     ctx.close()
@@ -708,7 +708,7 @@ def destruction_leg(args, threads):
     thits = torch.empty(train.records() * 16, dtype=torch.uint8, device="cuda")
     tsamp = torch.empty(train.slots * 96, dtype=torch.uint8, device="cuda")
     sharc = A.Sharc(20)
-    nrc = A.Nrc(A.NrcParams.default(bbox_size=64.0, log2_table=19))
+    nrc = A.Nrc(A.NrcParams.default(bbox_size=64.0, log2_table=14))
     shade = A.ShadeParams.default()
     rays0 = A.view_rays(cam, 1920, 1080, 250.0)
     n = frame.slots

@@ -160,7 +160,7 @@ def test_learned_cache_full_size(ctx):
     ctx.upload(*w.pools())
     del w
     run = LC.CacheRun(ctx, full_cam(), 1920, 1080, train_size=(344, 192), sharc_log2=20,
-                      nrc=A.NrcParams.default(bbox_size=64.0, log2_table=19), seed=1)
+                      nrc=A.NrcParams.default(bbox_size=64.0, log2_table=14), seed=1)
     assert run.n_train >= 1 << 16 and run.n_eval >= 1920 * 1080
     print(f"[config] learned cache: {run.n_train} training paths per frame, judge 1024 spp", flush=True)
     run.reference(1024)
