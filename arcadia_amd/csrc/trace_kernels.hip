@@ -309,8 +309,10 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
 // workgroup), or one workgroup of 1024 lanes per CU holding the roots of worlds up to 4096 chunks
 // (80 KB) beside its 64 KB of cold slots; both keep the 4 waves per SIMD the kernel's registers
 // allow.
-// WATER = false compiles the ignore_water tests out (neither ray class sets it).
-template <uint32_t BLOCK, uint32_t NCH, bool WATER, bool OFF32>
+// WATER = false compiles the ignore_water tests out (neither ray class sets it). UNI: the classes
+// trace alike apart from ignore_water (params_uniform), so the iteration reads the primary class's
+// scalar parameters and only the water flag is per lane.
+template <uint32_t BLOCK, uint32_t NCH, bool WATER, bool OFF32, bool UNI>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path(
     DevWorld w, DevParams p0, DevParams pb, PathCtl pc, uint4* __restrict__ out, uint32_t* __restrict__ queue,
     uint32_t refill, uint32_t batch) {
@@ -328,16 +330,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   uint32_t wb = 0u, we = 0u;
   uint32_t qc = home_counter(), qtried = 0u;
   bool drained = false;
+  // The finish / refill block runs when nidle >= thr (wave-uniform): thr = refill while slots are
+  // left to claim; once they are drained, one more than the idle lanes that hold nothing, so the
+  // block runs as soon as a ray finishes (its bounce starts at once). An ordinary trip costs one
+  // ballot and one scalar compare beyond the iteration.
+  uint32_t thr = refill;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     bool live = !idle_lane;  // tested again only after a finish / refill
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
-    const bool pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1)) != 0ull;
-    // drained, nothing pending, every lane idle; a uniform exit for the compiler (as in k_trace)
-    if (rfl((uint32_t)(drained && !pending && nidle == 64u))) break;
-    if (drained ? pending : nidle >= refill) {
+    if (rfl((uint32_t)(nidle >= thr))) {
+      const uint64_t pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1) != 0u);
+      // drained, nothing pending, every lane idle: done (uniform exit)
+      if (rfl((uint32_t)(drained && nidle == 64u && pending == 0ull))) break;
       if (idle_lane && fget(L.st, F_HASRAY, 1)) path_finish(w, pb, pc, out, L, tmax, C);
-      if (!drained) {
+      if (!drained && nidle >= refill) {
         const bool needs = fget(L.st, F_LIVE, 1) == 0u && fget(L.st, F_HASRAY, 1) == 0u;
         const uint64_t nm = __ballot(needs);
         const uint32_t nneed = (uint32_t)__popcll(nm);
@@ -356,12 +363,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         if (my != 0xFFFFFFFFu) path_start(w, p0, pc, my, out, L, tmax, C);
       }
       live = fget(L.st, F_LIVE, 1) != 0u;
+      const uint32_t nfree = (uint32_t)__popcll(__ballot(!live && fget(L.st, F_HASRAY, 1) == 0u));
+      thr = rfl(drained ? umin_(nfree + 1u, 64u) : refill);  // all 64 free: the next trip exits
     }
     // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
     // straight on.
-    if (live)
-      iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, sel_params(fget(L.st, F_DEPTH, 3) != 0u, pb, p0), L, C,
-                                                         s_roots, lds_roots);
+    if (live) {
+      const bool b = fget(L.st, F_DEPTH, 3) != 0u;
+      if (UNI) {
+        DevParams pl = p0;
+        if (WATER) pl.ignore_water = b ? pb.ignore_water : p0.ignore_water;
+        iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, pl, L, C, s_roots, lds_roots);
+      } else {
+        iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, sel_params(b, pb, p0), L, C, s_roots, lds_roots);
+      }
+    }
   }
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
 }
@@ -625,20 +641,29 @@ hipError_t launch_trace(const LaunchArgs& a, hipStream_t stream) {
 
 static const void* path_kernel(uint32_t block_threads) {
   return block_threads == kPathLargeBlock
-             ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge, true, true>)
-             : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks, true, true>);
+             ? reinterpret_cast<const void*>(&k_path<kPathLargeBlock, kLdsChunksLarge, true, true, false>)
+             : reinterpret_cast<const void*>(&k_path<kMaxBlock, kLdsChunks, true, true, false>);
 }
+
+static bool params_uniform(const DevParams& p0, const DevParams& pb);
 
 template <uint32_t BLOCK, uint32_t NCH, bool WATER>
 static void launch_path_as(const PathLaunchArgs& a, const DevWorld& w, const PathCtl& pc, hipStream_t stream) {
   const DevParams p0 = make_params(a.primary), pb = make_params(a.bounce);
   uint4* const out = reinterpret_cast<uint4*>(a.out);
-  if (a.off32)
-    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, true>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p0, pb,
-                       pc, out, a.queue, a.refill, a.batch);
+  const bool uni = params_uniform(p0, pb);
+  if (a.off32 && uni)
+    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, true, true>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                       p0, pb, pc, out, a.queue, a.refill, a.batch);
+  else if (a.off32)
+    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, true, false>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                       p0, pb, pc, out, a.queue, a.refill, a.batch);
+  else if (uni)
+    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, false, true>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                       p0, pb, pc, out, a.queue, a.refill, a.batch);
   else
-    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, false>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w, p0, pb,
-                       pc, out, a.queue, a.refill, a.batch);
+    hipLaunchKernelGGL((k_path<BLOCK, NCH, WATER, false, false>), dim3(a.blocks), dim3(a.block_threads), 0, stream, w,
+                       p0, pb, pc, out, a.queue, a.refill, a.batch);
 }
 
 hipError_t path_occupancy(uint32_t block_threads, uint32_t* blocks_per_cu) {
