@@ -153,13 +153,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
   uint32_t wb = 0u, we = 0u;  // wave-uniform: window of rays being handed out
   uint32_t qc = home_counter(), qtried = 0u;
   bool drained = false;
+  // The refill block runs when nidle >= thr (wave-uniform): thr = refill until the queue is
+  // drained, then 64, when the wave leaves; an ordinary trip costs one ballot and one scalar
+  // compare beyond the iteration.
+  uint32_t thr = refill;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
-    if (drained) {
-      if (nidle == 64u) break;
-    } else if (nidle >= refill) {
+    if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
+      if (rfl((uint32_t)drained)) break;
       if (idle_lane && fget(L.st, F_HASRAY, 1)) {
         shadow_finish<MODE>(vis, rec, p, L);
         L.st = fput(L.st, F_HASRAY, 1, 0u);
@@ -183,6 +186,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
         const arc_ray r = rays[my];
         ray_init<ARC_SEM_WORLD, true>(w, p, r, L, C);  // sets mat = kEntered past the entry test
       }
+      thr = rfl(drained ? 64u : refill);
     }
     // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
     // `continue` back to the ballot made a second back-edge, and the register allocator then

@@ -151,16 +151,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
   uint32_t qc = home_counter(), qtried = 0u;
   __shared__ float s_miss[kMaxBlock];  // t_max * 16 of each lane's ray (compact world records)
   bool drained = false;
+  // The refill block runs when nidle >= thr (wave-uniform): thr = refill until the queue is
+  // drained, then 64, when the wave leaves (its finished lanes are written after the loop). An
+  // ordinary trip costs one ballot and one scalar compare beyond the iteration; the exit is taken
+  // under a readfirstlane'd condition, so the compiler sees it uniform (an exit it took for
+  // divergent kept per-lane exit masks and copied every loop-carried lane register twice per trip).
+  uint32_t thr = refill;
   for (;;) {
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     bool live = !idle_lane;  // tested again only after a refill
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
-    // The exit test goes through readfirstlane so the compiler sees a uniform loop exit: with an
-    // exit it takes for divergent, it keeps per-lane exit masks and copies every loop-carried
-    // lane register twice per wave trip (14 extra vector moves per trip).
-    if (rfl((uint32_t)(drained && nidle == 64u))) break;
-    if (!drained && nidle >= refill) {
+    if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
+      if (rfl((uint32_t)drained)) break;
       // finalize finished lanes together, then refill every idle lane
       if (idle_lane && fget(L.st, F_HASRAY, 1)) {
         finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
@@ -186,6 +189,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
         ray_init<SEM>(w, p, r, L, C);
       }
       live = fget(L.st, F_LIVE, 1) != 0u;
+      thr = rfl(drained ? 64u : refill);
     }
     // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
     // `continue` back to the ballot made a second back-edge, and the register allocator then
@@ -339,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     bool live = !idle_lane;  // tested again only after a finish / refill
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
-    if (rfl((uint32_t)(nidle >= thr))) {
+    if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
       const uint64_t pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1) != 0u);
       // drained, nothing pending, every lane idle: done (uniform exit)
       if (rfl((uint32_t)(drained && nidle == 64u && pending == 0ull))) break;
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;  // pending lanes are not live
     bool live = !idle_lane;
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
-    if (rfl((uint32_t)(nidle >= thr))) {
+    if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
       const uint64_t hold = __ballot(idle_lane && (L.st & kHold) != 0u);
       // nothing left to claim, every lane idle and nothing held: done (uniform exit)
       if (rfl((uint32_t)(drained_b && nidle == 64u && hold == 0ull))) break;
