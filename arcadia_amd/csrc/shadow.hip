@@ -158,6 +158,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
   // compare beyond the iteration.
   uint32_t thr = refill;
   for (;;) {
+    // the iteration first, then the refill block (this order keeps the lane state in its loop
+    // registers without per-trip copies); lanes started there iterate from the next trip
+    if (fget(L.st, F_LIVE, 1)) shadow_iteration<MODE>(w, p, L, C, s_roots, lds_roots);
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
@@ -188,10 +191,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
       }
       thr = rfl(drained ? 64u : refill);
     }
-    // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
-    // `continue` back to the ballot made a second back-edge, and the register allocator then
-    // copied the whole lane state on every wave trip.)
-    if (fget(L.st, F_LIVE, 1)) shadow_iteration<MODE>(w, p, L, C, s_roots, lds_roots);
+    // One back-edge: the lanes refilled above begin at the top of the next trip. (A `continue`
+    // back to the ballot made a second back-edge, and the register allocator then copied the
+    // whole lane state on every wave trip.)
   }
   if (fget(L.st, F_HASRAY, 1)) shadow_finish<MODE>(vis, rec, p, L);
   queue_finish(queue, threadIdx.x == 0u);

@@ -82,7 +82,7 @@ __device__ __forceinline__ void iteration(const DevWorld& w, const DevParams& p,
     }
     const bool skip_water = SEM == ARC_SEM_WORLD && WATER && p.ignore_water && mat == ARC_MAT_WATER;
     hit = !skip_water && mat != ARC_MAT_AIR;  // hit: resolved at finalize (:467-523)
-    if (hit) L.mat = mat;
+    if (hit) st = fput(st, F_MAT, 8, mat);
   }
   // One way out of the iteration (a hit returns no earlier), so the lane state has one
   // definition per field where the iteration ends.
@@ -161,7 +161,7 @@ __device__ __forceinline__ uint4 compact_record(const arc_ray* rays, const Lane&
     const uint32_t face = h.nrm_axis > 0.0f ? 2u * h.axis : (h.nrm_axis < 0.0f ? 2u * h.axis + 1u : 6u + h.axis);
     v.x = __float_as_uint(h.exact_t);
     v.y = (L.vx & 0xFFFFu) | (L.vy << 16);
-    v.z = (L.vz & 0xFFFFu) | ((L.mat & 0xFFu) << 16) | (face << 24);
+    v.z = (L.vz & 0xFFFFu) | (fget(L.st, F_MAT, 8) << 16) | (face << 24);
     v.w = steps | ((ARC_HIT_FLAG_HIT | (fget(L.st, F_SHIFT, 3) > 0u ? ARC_HIT_FLAG_LOD : 0u)) << 16);
   } else {
     v.x = __float_as_uint(SEM == ARC_SEM_WORLD ? miss_t : 0.0f);

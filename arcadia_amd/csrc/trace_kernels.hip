@@ -54,7 +54,7 @@ __device__ __forceinline__ void store_full(arc_trace_result* out, const arc_ray*
     n0 = h.axis == 0 ? h.nrm_axis : 0.0f;
     n1 = h.axis == 1 ? h.nrm_axis : 0.0f;
     n2 = h.axis == 2 ? h.nrm_axis : 0.0f;
-    mat = L.mat;
+    mat = fget(L.st, F_MAT, 8);
     hitw = 1u | (h.axis << 8) | (sh << 16);
     if (SEM == ARC_SEM_WORLD) {
       hp0 = h.hx * 0.0625f;
@@ -151,15 +151,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
   uint32_t qc = home_counter(), qtried = 0u;
   __shared__ float s_miss[kMaxBlock];  // t_max * 16 of each lane's ray (compact world records)
   bool drained = false;
-  // The refill block runs when nidle >= thr (wave-uniform): thr = refill until the queue is
-  // drained, then 64, when the wave leaves (its finished lanes are written after the loop). An
-  // ordinary trip costs one ballot and one scalar compare beyond the iteration; the exit is taken
-  // under a readfirstlane'd condition, so the compiler sees it uniform (an exit it took for
-  // divergent kept per-lane exit masks and copied every loop-carried lane register twice per trip).
+  // A trip runs the iteration of the live lanes, then the refill block when nidle >= thr
+  // (wave-uniform): thr = refill until the queue is drained, then 64, when the wave leaves (its
+  // finished lanes are written after the loop). An ordinary trip costs one ballot and one scalar
+  // compare beyond the iteration; the exit is taken under a readfirstlane'd condition, so the
+  // compiler sees it uniform (an exit it took for divergent kept per-lane exit masks and copied
+  // every loop-carried lane register twice per trip). The iteration comes first in the loop body:
+  // with the refill block first, the lane state merged from the refill and the iteration was
+  // copied back into its loop registers on every trip (8 vector moves); this order needs 3.
   uint32_t thr = refill;
   for (;;) {
+    if (fget(L.st, F_LIVE, 1)) iteration<SEM, WATER, kMaxBlock, kLdsChunks, OFF32>(w, p, L, C, s_roots, lds_roots);
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
-    bool live = !idle_lane;  // tested again only after a refill
     const uint64_t idle = __ballot(idle_lane);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
@@ -188,13 +191,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? 1 : 
         if (!FULL && SEM == ARC_SEM_WORLD) s_miss[threadIdx.x] = r.t_max * 16.0f;
         ray_init<SEM>(w, p, r, L, C);
       }
-      live = fget(L.st, F_LIVE, 1) != 0u;
       thr = rfl(drained ? 64u : refill);
     }
-    // One back-edge: after a refill the live lanes, fresh ones included, go straight on. (A
-    // `continue` back to the ballot made a second back-edge, and the register allocator then
+    // One back-edge: the lanes refilled here begin their iteration at the top of the next trip.
+    // (A `continue` back to the ballot made a second back-edge, and the register allocator then
     // copied the whole lane state on every wave trip.)
-    if (live) iteration<SEM, WATER, kMaxBlock, kLdsChunks, OFF32>(w, p, L, C, s_roots, lds_roots);
   }
   if (fget(L.st, F_HASRAY, 1)) finalize<SEM, FULL>(out, rays, p, L, C, s_miss[threadIdx.x]);
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
@@ -285,7 +286,7 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
     exact_t = h.exact_t;
     v.x = __float_as_uint(h.exact_t);
     v.y = (L.vx & 0xFFFFu) | (L.vy << 16);
-    v.z = (L.vz & 0xFFFFu) | ((L.mat & 0xFFu) << 16) | (face << 24);
+    v.z = (L.vz & 0xFFFFu) | (fget(L.st, F_MAT, 8) << 16) | (face << 24);
     v.w = steps | ((ARC_HIT_FLAG_HIT | (fget(L.st, F_SHIFT, 3) > 0u ? ARC_HIT_FLAG_LOD : 0u)) << 16);
   } else {
     v.x = __float_as_uint(tmax * 16.0f);
@@ -340,8 +341,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
   // ballot and one scalar compare beyond the iteration.
   uint32_t thr = refill;
   for (;;) {
+    // the iteration first, then the finish / refill block (this order keeps the lane state in its
+    // loop registers without per-trip copies); lanes started there iterate from the next trip
+    if (fget(L.st, F_LIVE, 1)) {
+      const bool b = fget(L.st, F_DEPTH, 3) != 0u;
+      if (UNI) {
+        DevParams pl = p0;
+        if (WATER) pl.ignore_water = b ? pb.ignore_water : p0.ignore_water;
+        iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, pl, L, C, s_roots, lds_roots);
+      } else {
+        iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, sel_params(b, pb, p0), L, C, s_roots, lds_roots);
+      }
+    }
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
-    bool live = !idle_lane;  // tested again only after a finish / refill
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
     if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
       const uint64_t pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1) != 0u);
@@ -366,21 +378,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
         }
         if (my != 0xFFFFFFFFu) path_start(w, p0, pc, my, out, L, tmax, C);
       }
-      live = fget(L.st, F_LIVE, 1) != 0u;
+      const bool live = fget(L.st, F_LIVE, 1) != 0u;
       const uint32_t nfree = (uint32_t)__popcll(__ballot(!live && fget(L.st, F_HASRAY, 1) == 0u));
       thr = rfl(drained ? umin_(nfree + 1u, 64u) : refill);  // all 64 free: the next trip exits
-    }
-    // One back-edge: lanes live after the finish / refill (bounces and fresh slots included) go
-    // straight on.
-    if (live) {
-      const bool b = fget(L.st, F_DEPTH, 3) != 0u;
-      if (UNI) {
-        DevParams pl = p0;
-        if (WATER) pl.ignore_water = b ? pb.ignore_water : p0.ignore_water;
-        iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, pl, L, C, s_roots, lds_roots);
-      } else {
-        iteration<ARC_SEM_WORLD, WATER, BLOCK, NCH, OFF32>(w, sel_params(b, pb, p0), L, C, s_roots, lds_roots);
-      }
     }
   }
   queue_finish(queue, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)
@@ -495,8 +495,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
   // ordinary trip costs one ballot and one scalar compare beyond the iteration.
   uint32_t thr = refill, tick = 0u;
   for (;;) {
+    // the iteration first, then the refill block (this order keeps the lane state in its loop
+    // registers without per-trip copies); lanes started there iterate from the next trip
+    if (fget(L.st, F_LIVE, 1)) {
+      const bool b = fget(L.st, F_DEPTH, 3) != 0u;
+      if (UNI) {
+        DevParams pl = p0;
+        if (WATER) pl.ignore_water = b ? pb.ignore_water : p0.ignore_water;
+        iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(w, pl, L, C, s_roots, lds_roots);
+      } else {
+        iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(w, sel_params(b, pb, p0), L, C, s_roots,
+                                                                      lds_roots);
+      }
+    }
     const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;  // pending lanes are not live
-    bool live = !idle_lane;
     const uint32_t nidle = (uint32_t)__popcll(__ballot(idle_lane));
     if (nidle >= thr) {  // uniform: a ballot count against a uniform threshold
       const uint64_t hold = __ballot(idle_lane && (L.st & kHold) != 0u);
@@ -556,23 +568,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
           }
         }
       }
-      live = fget(L.st, F_LIVE, 1) != 0u;
+      const bool live = fget(L.st, F_LIVE, 1) != 0u;
       const bool pend = __ballot(fget(L.st, F_PEND, 1) != 0u) != 0ull;
       const bool prim = drained_b && __ballot((L.st & (1u << F_HASRAY)) != 0u && fget(L.st, F_DEPTH, 3) == 0u) != 0ull;
       thr = rfl((pend || prim) ? 0u : (drained_b ? 64u : refill));
       // nothing to trace and a message still missing: poll more slowly (~500 cycles)
       if (rfl((uint32_t)(pend && __ballot(live) == 0ull))) __builtin_amdgcn_s_sleep(8);
-    }
-    if (live) {
-      const bool b = fget(L.st, F_DEPTH, 3) != 0u;
-      if (UNI) {
-        DevParams pl = p0;
-        if (WATER) pl.ignore_water = b ? pb.ignore_water : p0.ignore_water;
-        iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(w, pl, L, C, s_roots, lds_roots);
-      } else {
-        iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(w, sel_params(b, pb, p0), L, C, s_roots,
-                                                                      lds_roots);
-      }
     }
   }
   queue_finish2(queue, queue_b, C == s_cold);  // thread 0 (C = s_cold + threadIdx.x)

@@ -17,6 +17,7 @@ enum : uint32_t {
   F_HIT = 7,     // 1 bit: finished with a hit
   F_HASRAY = 8,  // 1 bit: lane owns a ray whose record is not written yet
   F_DEPTH = 9,   // 3 bits: path kernel: 0 = primary ray, d = d-th bounce
+  F_MAT = 16,    // 8 bits: traversal kernels: material of the hit (valid when done with a hit)
   F_LIVE = 31,   // 1 bit: ray in flight (the sign bit: the per-trip test is one compare)
 };
 
@@ -38,7 +39,7 @@ struct Lane {
   uint32_t step;     // outer iterations begun
   uint32_t st;       // packed flags
   uint32_t cs;       // cached_shift of the reference: the level the last descent ended on
-  uint32_t mat;      // material of the hit
+  uint32_t mat;      // shadow walkers: entry marker / outcome (the traversal keeps the hit material in st)
   uint4 node;        // node of the level the last descent ended on
   uint32_t idx;
 };
@@ -293,8 +294,14 @@ __device__ __forceinline__ void box_step(Lane& L, uint32_t span) {
   const uint32_t nvx = is_x ? fx : cx;
   const uint32_t nvy = is_y ? fy : cy;
   const uint32_t nvz = (is_x || is_y) ? cz : fz;
+  // `diff` is read only through its highest set bit (restart_shift, the chunk-change test), and
+  // (nvx ^ bx) | (nvy ^ by) | (nvz ^ bz) has the same highest bit as the XOR of the old and new
+  // voxels: on the other axes the new voxel stays inside the box (old and new share the bits at
+  // and above log2(span) with b), and on the exit axis the old voxel and b differ only below
+  // log2(span) while f ^ b has a bit at or above it. Written with b instead of the old voxel,
+  // the old coordinates are dead once b is formed, so the new ones can take their registers.
   // (a ^ b) | c as one three-input bit operation (truth table 0xBE)
-  L.diff = __builtin_amdgcn_bitop3_b32(nvz, L.vz, __builtin_amdgcn_bitop3_b32(nvy, L.vy, nvx ^ L.vx, 0xBE), 0xBE);
+  L.diff = __builtin_amdgcn_bitop3_b32(nvz, bz, __builtin_amdgcn_bitop3_b32(nvy, by, nvx ^ bx, 0xBE), 0xBE);
   L.vx = nvx;
   L.vy = nvy;
   L.vz = nvz;
