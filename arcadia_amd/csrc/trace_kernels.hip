@@ -424,7 +424,7 @@ __device__ __forceinline__ uint64_t msg_load(uint64_t* m) {
 
 // Bounce index i's message `m` (non-zero) taken: the slot's empty record, or the bounce ray set
 // up in the lane. `frames`: arcmath::bounce_frame of the nine face codes.
-__device__ __forceinline__ void bounce_take(const DevWorld& w, const DevParams& pb, const arc_ray* rays, uint64_t m,
+__device__ __forceinline__ void bounce_take(const DevWorld& w, const DevParams& pb, const arc_ray& pr, uint64_t m,
                                             uint64_t seed, float bounce_tmax, uint4* out_b, Lane& L, uint4* C,
                                             const arcmath::BounceFrame* frames) {
   const uint32_t i = L.idx;
@@ -433,7 +433,6 @@ __device__ __forceinline__ void bounce_take(const DevWorld& w, const DevParams& 
     L.st = 0u;
     return;
   }
-  const arc_ray pr = rays[i];
   const float ov[3] = {pr.ox * 16.0f, pr.oy * 16.0f, pr.oz * 16.0f}, dr[3] = {pr.dx, pr.dy, pr.dz};
   const uint32_t face = (uint32_t)(m >> 32) - 1u;
   arc_ray r;
@@ -561,10 +560,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
           }
         }
         if (fget(L.st, F_PEND, 1)) {
+          // the parent ray is loaded beside the message (one memory round trip instead of two)
+          const arc_ray pr = rays[L.idx];
           const uint64_t m = msg_load(msg + L.idx);
           if (m != 0ull) {
             msg_store(msg + L.idx, 0ull);
-            bounce_take(w, pb, rays, m, seed, bounce_tmax, out_b, L, C, s_frame);
+            bounce_take(w, pb, pr, m, seed, bounce_tmax, out_b, L, C, s_frame);
           }
         }
       }

@@ -997,16 +997,18 @@ static void trace_chunk_one(const orc_world* w, uint32_t root, const float o[3],
 }
 
 /* ---- WorldMap::raycast_meters: world_map.rs:464-545 --------------------------------------
- * glam's normalize_or_zero is restated as v * (1/sqrt(v.v)) with the dot product summed
- * x,y,z left to right; floor() of a positive value (p/256) as floorf. */
+ * glam 0.29.3 (Cargo.lock), Vec3::normalize_or_zero = normalize_or(ZERO): rcp = 1.0 / length(),
+ * length = sqrt(dot) with dot = (x*x + y*y) + z*z; the result is self * rcp when rcp is finite
+ * and > 0, else zero (then the reference returns None). floor() of a positive value (p/256) as
+ * floorf. */
 static void trace_meters_one(const orc_world* w, const arc_ray* ray, arc_trace_result* res) {
   memset(res, 0, sizeof(*res));
   res->contact_ao = 1.0f;
   float d0[3] = {ray->dx, ray->dy, ray->dz};
-  float dd = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2];
+  float dd = (d0[0] * d0[0] + d0[1] * d0[1]) + d0[2] * d0[2];
   float rl = 1.0f / sqrtf(dd);
   float d[3];
-  if (!(rl < INFINITY) || rl != rl) return; /* normalize_or_zero -> zero -> None */
+  if (!(rl > 0.0f && rl < INFINITY)) return; /* normalize_or_zero -> zero -> None */
   for (int a = 0; a < 3; ++a) d[a] = d0[a] * rl;
   if (d[0] == 0.0f && d[1] == 0.0f && d[2] == 0.0f) return;
   float vo[3] = {ray->ox * 16.0f, ray->oy * 16.0f, ray->oz * 16.0f};

@@ -136,3 +136,41 @@ def test_kats_on_gpu(ctx):
         ref = O.compact(O.trace_chunk(O.World(*pools), rc)[0])
         assert got.tobytes() == ref.tobytes()
         assert got["flags"][0] & 1
+
+
+def test_meters_direction_follows_glam_normalize_or_zero():
+    """raycast_meters normalizes its direction with glam 0.29.3's Vec3::normalize_or_zero
+    (world_map.rs:470, Cargo.lock pins glam 0.29.3): rcp = 1 / length(); self * rcp when rcp is
+    finite and > 0, else ZERO, and a zero direction returns None (:471-473). A zero, NaN or
+    infinite direction and one whose squared length overflows therefore miss; a scaled direction
+    traces exactly as the unit direction the rule produces."""
+    pools = _floor_world()
+    ow = O.World(*pools)
+    o = (8.0, 3.0, 8.0)  # metres: 48 voxels up, the floor's top face is at y = 32 voxels
+    unit = O.trace_meters(ow, _ray(o, (0.0, -1.0, 0.0), 10.0))[0]
+    assert unit["hit"] and unit["vy"] == 31 and unit["mat"] == MAT_STONE_DEEP
+    assert abs(float(unit["t_voxels"]) - 16.0) < 0.05  # 16 voxels (1 m) to the top face
+    for d in ((0.0, 0.0, 0.0), (np.nan, -1.0, 0.0), (0.0, -np.inf, 0.0), (1e20, -1e20, 0.0)):
+        r = O.trace_meters(ow, _ray(o, d, 10.0))[0]
+        assert not r["hit"], d
+    # (1e19, -1e19, 0): the squares stay finite (2e38 < FLT_MAX), so it normalizes and hits
+    assert O.trace_meters(ow, _ray(o, (1e19, -1e19, 0.0), 10.0))[0]["hit"]
+    # a scaled direction gives the record of the normalized one, bit for bit: by a power of two
+    # always (d * 2^k scales the dot product by 4^k, its root by 2^k and rcp by 2^-k, all exact),
+    # by other factors whenever the rule maps both to the same unit vector
+    rng = np.random.default_rng(7)
+    same = 0
+    for _ in range(50):
+        d = rng.normal(size=3).astype(np.float32)
+        d[1] = -abs(d[1]) - np.float32(0.2)
+        raw = O.trace_meters(ow, _ray(o, tuple(d), 50.0))
+        assert raw[0]["hit"]
+        for scale in (np.float32(0.25), np.float32(8.0)):
+            a = O.trace_meters(ow, _ray(o, tuple(np.float32(x * scale) for x in d), 50.0))
+            assert a.tobytes() == raw.tobytes()
+        for scale in (np.float32(0.37), np.float32(5.0)):
+            if _glam_normalize(tuple(np.float32(x * scale) for x in d)) == _glam_normalize(d):
+                same += 1
+                a = O.trace_meters(ow, _ray(o, tuple(np.float32(x * scale) for x in d), 50.0))
+                assert a.tobytes() == raw.tobytes()
+    assert same >= 5
