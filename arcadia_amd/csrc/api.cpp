@@ -194,7 +194,7 @@ int arc_create(int device, arc_ctx** out) {
   c->cfg_bounce = default_config();
   c->cfg_bounce.blocks_per_cu = 0;
   c->cfg_bounce.refill_lanes = 24;
-  c->cfg_bounce.batch_rays = 32;
+  c->cfg_bounce.batch_rays = 64;
   if (hipSetDevice(device) != hipSuccess) {
     delete c;
     return ARC_ERR_HIP;

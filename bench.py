@@ -682,7 +682,7 @@ def cache_leg(args, threads):
 def destruction_leg(args, threads):
     """30 frames over the full-frame world; each frame carves >= 1% of the solid voxels with
     seeded spheres (radius 4..32 voxels), syncs the touched nodes into GPU patches, traces the
-    1080p primary + diffuse frame in one launch, checks a sample of it bit-exact against the
+    1080p primary + diffuse frame in one launch (arc_trace_bounce_device), checks a sample of it bit-exact against the
     oracle on the edited host pools, invalidates SHARC surfels in the carved spheres and takes a
     neural-cache training step on fresh paths."""
     import torch
@@ -698,20 +698,20 @@ def destruction_leg(args, threads):
     world.clear_patches()
     cam = A.camera_look((31.3, 58.0, 27.1), 0.7, -0.5, 65.0, 1920, 1080)
     p0, pb = A.TraceParams.exact(0), A.TraceParams.exact(2, ignore_water=True)
-    frame = ctx.batch([(cam, 1920, 1080, 250.0)], p0, pb, n_bounces=1, bounce_max_dist_m=26.0, seed=args.seed)
     tcam = A.camera_look((0.0, 0.0, 0.0), 0.0, 0.0, 60.0, 341, 192)
     for f in ("eye", "forward", "right", "up"):
         setattr(tcam, f, getattr(cam, f))
     tcam.tan_half_fov, tcam.aspect = cam.tan_half_fov, 341 / 192
     train = ctx.batch([(tcam, 341, 192, 250.0)], p0, pb, n_bounces=4, bounce_max_dist_m=250.0, seed=args.seed)
-    hits = torch.empty(frame.records() * 16, dtype=torch.uint8, device="cuda")
+    rays0 = A.view_rays(cam, 1920, 1080, 250.0)
+    n = len(rays0)
+    d_rays0 = torch.from_numpy(rays0.view(np.uint8)).cuda()
+    hits = torch.empty(2 * n * 16, dtype=torch.uint8, device="cuda")
     thits = torch.empty(train.records() * 16, dtype=torch.uint8, device="cuda")
     tsamp = torch.empty(train.slots * 96, dtype=torch.uint8, device="cuda")
     sharc = A.Sharc(20)
     nrc = A.Nrc(A.NrcParams.default(bbox_size=64.0, log2_table=14))
     shade = A.ShadeParams.default()
-    rays0 = A.view_rays(cam, 1920, 1080, 250.0)
-    n = frame.slots
     rng = np.random.default_rng(args.seed + 5)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     t_edit, t_patch, t_trace, t_cache, carved, exact, patched_mb = [], [], [], [], [], 0, []
@@ -731,9 +731,9 @@ def destruction_leg(args, threads):
             patched_mb.append(sum(p["nodes"].nbytes + p["bricks"].nbytes for p in pl) / 2**20)
         ctx.patch_world(world)
         t2 = time.perf_counter()
-        frame.set_seed(args.seed * 7919 + f)
         ev[0].record()
-        frame.trace(hits.data_ptr())
+        # the frame's primaries (8x8-tile order) and their bounces in one launch
+        ctx.trace_bounce_device(p0, pb, d_rays0.data_ptr(), n, args.seed * 7919 + f, 26.0, hits.data_ptr())
         ev[1].record()
         sharc.invalidate(np.array(spheres, np.float32))
         train.set_seed(args.seed * 104729 + f)
@@ -786,7 +786,6 @@ def destruction_leg(args, threads):
     }
     sharc.close()
     nrc.close()
-    frame.close()
     train.close()
     ctx.close()
     return out
