@@ -486,6 +486,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
   uint32_t qc = home_counter(), qtried = 0u, qc_b = qc, qtried_b = 0u;
   bool drained_p = false, drained_b = false;
   constexpr uint32_t kHold = (1u << F_HASRAY) | (1u << F_PEND);
+  // Primary phase: every lane holds a primary ray or nothing, so the loop is k_trace's (finalize
+  // also publishes the message). It ends once the primary windows run dry, after the lanes of that
+  // refill are set up; the lanes' state carries over into the loop below, which also claims
+  // bounce indices. With one loop for both phases the primary half of the launch ran the bounce
+  // bookkeeping too (+23% scalar instructions).
+  {
+    for (;;) {
+      if (fget(L.st, F_LIVE, 1)) iteration<ARC_SEM_WORLD, WATER, kMaxBlock, kLdsChunks, OFF32>(
+          w, p0, L, C, s_roots, lds_roots);
+      const bool idle_lane = fget(L.st, F_LIVE, 1) == 0u;
+      const uint64_t idle = __ballot(idle_lane);
+      const uint32_t nidle = (uint32_t)__popcll(idle);
+      if (nidle >= refill) {  // uniform: a ballot count against a uniform threshold
+        if (idle_lane && fget(L.st, F_HASRAY, 1)) {
+          const uint4 v = compact_record<ARC_SEM_WORLD>(rays, L, s_miss[threadIdx.x]);
+          out[L.idx] = v;
+          msg_store(msg + L.idx, (v.w & ((uint32_t)ARC_HIT_FLAG_HIT << 16)) != 0u
+                                     ? ((uint64_t)(1u + (v.z >> 24)) << 32) | v.x
+                                     : kMsgMiss);
+          L.st = 0u;
+        }
+        const uint32_t rank = (uint32_t)__popcll(idle & lt_mask);
+        uint32_t assigned = 0u, my = 0xFFFFFFFFu;
+        while (assigned < nidle) {
+          if (wb >= we && !claim_window(queue, qc, qtried, batch, n, wb, we)) {
+            drained_p = true;
+            break;
+          }
+          const uint32_t take = umin_(nidle - assigned, we - wb);
+          if (idle_lane && rank >= assigned && rank < assigned + take) my = wb + (rank - assigned);
+          wb += take;
+          assigned += take;
+        }
+        if (my != 0xFFFFFFFFu) {
+          L.idx = my;
+          const arc_ray r = rays[my];
+          s_miss[threadIdx.x] = r.t_max * 16.0f;
+          ray_init<ARC_SEM_WORLD>(w, p0, r, L, C);
+        }
+        if (rfl((uint32_t)drained_p)) break;
+      }
+    }
+  }
   // The refill block runs when nidle >= thr (wave-uniform): thr = refill while claims go on, 64
   // once nothing is left to claim, and 0 (every trip) while a lane waits for a message or, with
   // nothing left to claim, while the wave still traces a primary (published at once when it
