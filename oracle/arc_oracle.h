@@ -22,9 +22,12 @@
  * seeded worlds; closed forms for axis-aligned rays; agreement between orc_trace_world and
  * orc_trace_chunk where the reference semantics coincide (one chunk, no LOD, no tiny
  * direction components); and a second, pure-Python restatement (oracle/py_trace.py) that
- * must match step counts bit for bit on small worlds. orc_trace_meters is pinned only by
- * the KATs and the brute-force walker (its glam arithmetic is restated as per-lane IEEE
- * ops; bit-level agreement with glam is unpinned).
+ * must match step counts bit for bit on small worlds; hand-derived step counts, LOD stops,
+ * lod_rgb and contact-AO closed forms on constructed worlds (tests/test_step_kat.py).
+ * orc_trace_meters is pinned by the KATs, the brute-force walker and the rule of glam
+ * 0.29.3's Vec3::normalize_or_zero (rcp = 1 / length; self * rcp when rcp is finite and
+ * > 0, else zero), checked on zero, NaN, infinite and overflowing directions and on
+ * power-of-two scalings (tests/test_kat.py); glam's compiled code cannot run here.
  */
 #ifndef ARC_ORACLE_H
 #define ARC_ORACLE_H
