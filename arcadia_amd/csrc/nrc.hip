@@ -296,13 +296,15 @@ __global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g0, const uint32_t* 
 }
 
 // Forward + backward of one batch; writes the activations and deltas ([feature][sample], FP16)
-// for the weight-gradient kernel, scatters the encoding gradient into the table gradient, and
-// accumulates the loss.
+// for the weight-gradient kernel and accumulates the loss. The encoding gradient either goes to
+// `enc_grad` ([level][sample] pairs, with the samples' grid positions in `enc_pos`) for
+// k_nrc_scatter, or, when enc_grad is null, straight into the table gradient with global atomics.
 __global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_t* __restrict__ table_h,
                                                       const _Float16* __restrict__ mlp_h,
                                                       const arc_path_sample* __restrict__ s, uint32_t n,
                                                       uint32_t ld, _Float16* __restrict__ act,
-                                                      float* __restrict__ table_grad, float* __restrict__ loss) {
+                                                      float* __restrict__ table_grad, float2* __restrict__ enc_grad,
+                                                      float4* __restrict__ enc_pos, float* __restrict__ loss) {
   __shared__ MlpLds L;
   __shared__ MlpLdsT T;
   __shared__ GridInfo sg;  // per-lane level lookups from LDS (as in k_nrc_infer)
@@ -331,6 +333,7 @@ __global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_
     const uint32_t col = tile * 16u + (lane & 15u);
     const uint32_t i = col;
     const SampleIn si = load_sample(s, i, n, g);
+    if (enc_pos && grp == 0u && i < n) enc_pos[i] = make_float4(si.p[0], si.p[1], si.p[2], si.valid ? 1.0f : 0.0f);
     half4 xb[3], a1[4], a2[4], a3[4];
     input_tiles(g, table_h, si, grp, xb);
     const floatx4 y = forward_tile(L, xb, lane, a1, a2, a3);
@@ -375,7 +378,13 @@ __global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_
     for (int mt = 0; mt < 2; ++mt) {
       floatx4 acc = z;
       for (int kt = 0; kt < 4; ++kt) acc = mfma(lds_a(T.w1t, kPadHid, mt, kt, lane), d1[kt], acc);
-      if (si.valid) {
+      if (enc_grad) {
+        if (i < n) {
+          const bool v = si.valid;
+          enc_grad[(size_t)(mt * 8 + 2 * grp) * ld + i] = make_float2(v ? acc[0] : 0.0f, v ? acc[1] : 0.0f);
+          enc_grad[(size_t)(mt * 8 + 2 * grp + 1) * ld + i] = make_float2(v ? acc[2] : 0.0f, v ? acc[3] : 0.0f);
+        }
+      } else if (si.valid) {
         scatter_level(g, table_grad, mt * 8 + 2 * grp, si.p, acc[0], acc[1]);
         scatter_level(g, table_grad, mt * 8 + 2 * grp + 1, si.p, acc[2], acc[3]);
       }
@@ -400,6 +409,69 @@ __global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_
   // loss: wave sum then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
   if (lane == 0 && lsum != 0.0f) atomicAdd(loss, lsum);
+}
+
+// Table-gradient scatter through LDS, for tables of at most kLdsScatterEntries entries per level.
+// Random FP32 atomics into device memory ran at about 21 G/s on MI355X, with the same rate from
+// 1024 to 16384 workgroups and with one table copy per XCD (measured); the batch's 16 levels x
+// 8 corners x 2 features per sample (16.9 M at 2^16 paths) took 0.70 of a 0.88 ms training step
+// that way. Here workgroup (level l, chunk c) adds the gradients of its samples' 8 corners at
+// level l into an LDS copy of that level's table with LDS atomics, then stores the copy to its
+// partial slice, and k_nrc_reduce adds the slices into the table gradient: no global atomics.
+constexpr uint32_t kLdsScatterEntries = 1u << 14;  // 128 KB of FP32 pairs per workgroup
+
+__global__ __launch_bounds__(256) void k_nrc_scatter(GridInfo g0, const float2* __restrict__ enc_grad,
+                                                     const float4* __restrict__ enc_pos, uint32_t n, uint32_t ld,
+                                                     uint32_t chunk, float* __restrict__ partial) {
+  __shared__ float tab[2 * kLdsScatterEntries];
+  __shared__ GridInfo sg;  // level lookups from LDS (a dynamically indexed kernel argument goes to the stack)
+  if (threadIdx.x == 0) sg = g0;
+  const uint32_t l = blockIdx.x % (uint32_t)kLevels, c = blockIdx.x / (uint32_t)kLevels;
+  const uint32_t words = 2u * (g0.table_mask + 1u);  // <= 2 * kLdsScatterEntries (checked by the host)
+  for (uint32_t j = threadIdx.x; j < words; j += blockDim.x) tab[j] = 0.0f;
+  __syncthreads();
+  const GridInfo& g = sg;
+  const uint32_t i0 = c * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    const float4 q = enc_pos[i];
+    const float2 d = enc_grad[(size_t)l * ld + i];
+    if (q.w == 0.0f || (d.x == 0.0f && d.y == 0.0f)) continue;
+    const float p[3] = {q.x, q.y, q.z};
+    const LevelPos lp = level_pos(g, (int)l, p);
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+      const float w = (dx ? lp.f[0] : 1.0f - lp.f[0]) * (dy ? lp.f[1] : 1.0f - lp.f[1]) * (dz ? lp.f[2] : 1.0f - lp.f[2]);
+      const uint32_t e = corner_index(g, (int)l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz);
+      atomicAdd(&tab[2u * e], w * d.x);
+      atomicAdd(&tab[2u * e + 1u], w * d.y);
+    }
+  }
+  __syncthreads();
+  float* out = partial + ((size_t)c * kLevels + l) * words;
+  for (uint32_t j = threadIdx.x; j < words; j += blockDim.x) out[j] = tab[j];
+}
+
+// grad[j] += sum over the nparts partial slices (each `words` floats, the gradient's layout).
+__global__ __launch_bounds__(256) void k_nrc_reduce(const float4* __restrict__ partial, uint32_t nparts, uint64_t words4,
+                                                    float4* __restrict__ grad) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < words4; j += (uint64_t)gridDim.x * blockDim.x) {
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t c = 0; c < nparts; ++c) {
+      const float4 v = partial[(size_t)c * words4 + j];
+      a.x += v.x;
+      a.y += v.y;
+      a.z += v.z;
+      a.w += v.w;
+    }
+    if (a.x != 0.0f || a.y != 0.0f || a.z != 0.0f || a.w != 0.0f) {
+      float4 gv = grad[j];
+      gv.x += a.x;
+      gv.y += a.y;
+      gv.z += a.z;
+      gv.w += a.w;
+      grad[j] = gv;
+    }
+  }
 }
 
 // Weight gradient of one layer over a chunk of samples: dW (rows x cols) += D (rows x s) . A^T
@@ -484,6 +556,13 @@ struct arc_nrc {
   _Float16* mlp_h = nullptr;
   _Float16* act = nullptr;  // training planes
   uint32_t act_ld = 0;
+  // LDS table-gradient scatter (tables of <= kLdsScatterEntries entries per level): encoding
+  // gradients [level][act_ld] and grid positions [act_ld] of a batch, and the workgroups' partial
+  // gradient slices (scatter_parts x grid_params floats)
+  float2* enc_grad = nullptr;
+  float4* enc_pos = nullptr;
+  float* scatter_part = nullptr;
+  uint32_t scatter_parts = 0;
   float* loss = nullptr;
   uint64_t step = 0;
   uint32_t n_cu = 0;
@@ -512,8 +591,9 @@ int nrc_alloc(arc_nrc* r) {
 }
 
 void nrc_free(arc_nrc* r) {
-  void* ps[] = {r->grid_p, r->grid_m, r->grid_v, r->grid_g, r->grid_h, r->mlp_p, r->mlp_m,
-                r->mlp_v,  r->mlp_g, r->mlp_h, r->act,    r->loss,   r->sort_buf, r->sort_tmp};
+  void* ps[] = {r->grid_p, r->grid_m, r->grid_v,  r->grid_g,   r->grid_h,   r->mlp_p,    r->mlp_m,
+                r->mlp_v,  r->mlp_g, r->mlp_h,   r->act,      r->loss,     r->sort_buf, r->sort_tmp,
+                r->enc_grad, r->enc_pos, r->scatter_part};
   for (void* p : ps) (void)hipFree(p);
 }
 
@@ -665,21 +745,44 @@ int arc_nrc_backward(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, v
   hipStream_t s = (hipStream_t)stream;
   if (hipSetDevice(r->device) != hipSuccess) return ARC_ERR_HIP;
   const uint32_t ld = (uint32_t)((n + 15) / 16 * 16);
+  const bool lds_scatter = r->table_entries <= kLdsScatterEntries;
   if (ld > r->act_ld) {
     if (hipStreamSynchronize(s) != hipSuccess) return ARC_ERR_HIP;
-    (void)hipFree(r->act);
+    void* old[] = {r->act, r->enc_grad, r->enc_pos};
+    for (void* p : old) (void)hipFree(p);
     r->act = nullptr;
+    r->enc_grad = nullptr;
+    r->enc_pos = nullptr;
     r->act_ld = 0;
     const size_t rows = kIn + 6 * kHid + kOut;
     if (hipMalloc(&r->act, rows * ld * 2) != hipSuccess) return ARC_ERR_HIP;
+    if (lds_scatter && (hipMalloc(&r->enc_grad, (size_t)kLevels * ld * sizeof(float2)) != hipSuccess ||
+                        hipMalloc(&r->enc_pos, (size_t)ld * sizeof(float4)) != hipSuccess))
+      return ARC_ERR_HIP;
     r->act_ld = ld;
+  }
+  if (lds_scatter && !r->scatter_part) {
+    const uint32_t parts = std::max<uint32_t>(1u, r->n_cu / (uint32_t)kLevels);  // one workgroup per CU
+    if (hipMalloc(&r->scatter_part, (size_t)parts * r->grid_params * 4) != hipSuccess) return ARC_ERR_HIP;
+    r->scatter_parts = parts;
   }
   const uint64_t tiles = ld / 16;
   const uint32_t blocks = (uint32_t)std::min<uint64_t>((tiles + 3) / 4, (uint64_t)r->n_cu * 4);
   // Training keeps the batch order: in Morton order the lanes of a tile hit the same coarse-level
-  // entries and their gradient atomics serialize (measured slower at 16K and 66K samples).
+  // entries, and with the global-atomic scatter those atomics serialize (measured slower at 16K
+  // and 66K samples).
   hipLaunchKernelGGL(k_nrc_train_fb, dim3(blocks), dim3(256), 0, s, r->g, reinterpret_cast<const uint32_t*>(r->grid_h),
-                     r->mlp_h, d_samples, (uint32_t)n, r->act_ld, r->act, r->grid_g, r->loss);
+                     r->mlp_h, d_samples, (uint32_t)n, r->act_ld, r->act, r->grid_g, lds_scatter ? r->enc_grad : nullptr,
+                     lds_scatter ? r->enc_pos : nullptr, r->loss);
+  if (lds_scatter) {
+    // chunks of at least 256 samples, at most one per partial slice
+    const uint32_t parts = (uint32_t)std::min<uint64_t>(r->scatter_parts, std::max<uint64_t>(1, (n + 255) / 256));
+    const uint32_t chunk = (uint32_t)((n + parts - 1) / parts);
+    hipLaunchKernelGGL(k_nrc_scatter, dim3(parts * (uint32_t)kLevels), dim3(256), 0, s, r->g, r->enc_grad, r->enc_pos,
+                       (uint32_t)n, r->act_ld, chunk, r->scatter_part);
+    hipLaunchKernelGGL(k_nrc_reduce, dim3(2048), dim3(256), 0, s, reinterpret_cast<const float4*>(r->scatter_part),
+                       parts, r->grid_params / 4, reinterpret_cast<float4*>(r->grid_g));
+  }
   const uint32_t ldv = r->act_ld;
   _Float16* X = r->act;
   _Float16* A1 = X + (size_t)kIn * ldv;

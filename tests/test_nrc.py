@@ -83,11 +83,13 @@ def test_level_resolutions_and_dense_levels():
 
 
 @pytest.mark.gpu
-def test_gpu_infer_and_gradients_match_restatement(ctx):
+@pytest.mark.parametrize("log2", [14, 16])
+def test_gpu_infer_and_gradients_match_restatement(ctx, log2):
+    """2^14 entries per level: the table gradient is accumulated per level in LDS and reduced
+    (k_nrc_scatter, k_nrc_reduce); 2^16: global atomics. Both add to the gradient buffers."""
     import torch
 
     rng = np.random.default_rng(1)
-    log2 = 14
     nrc = A.Nrc(A.NrcParams.default(log2_table=log2))
     scale, dense = nrc.scale, nrc.dense
     assert np.array_equal(scale, level_info(log2)[0]) and np.array_equal(dense, level_info(log2)[1])
@@ -125,6 +127,13 @@ def test_gpu_infer_and_gradients_match_restatement(ctx):
         err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
         print(f"[nrc] {name}: rel err {err:.2e}", flush=True)
         assert err < 1e-2, name
+    # a second backward adds to the gradients
+    nrc.backward(d.data_ptr(), n)
+    _, _, gg2, mg2 = nrc.get()
+    nrc.loss(reset=True)
+    for name, a, b in (("table x2", gg2, 2.0 * gg), ("MLP x2", mg2, 2.0 * mg)):
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert err < 1e-5, (name, err)
     nrc.close()
 
 
