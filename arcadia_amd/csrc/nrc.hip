@@ -299,12 +299,13 @@ __global__ __launch_bounds__(256) void k_nrc_infer(GridInfo g0, const uint32_t* 
 // for the weight-gradient kernel and accumulates the loss. The encoding gradient either goes to
 // `enc_grad` ([level][sample] pairs, with the samples' grid positions in `enc_pos`) for
 // k_nrc_scatter, or, when enc_grad is null, straight into the table gradient with global atomics.
-__global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_t* __restrict__ table_h,
-                                                      const _Float16* __restrict__ mlp_h,
-                                                      const arc_path_sample* __restrict__ s, uint32_t n,
-                                                      uint32_t ld, _Float16* __restrict__ act,
-                                                      float* __restrict__ table_grad, float2* __restrict__ enc_grad,
-                                                      float4* __restrict__ enc_pos, float* __restrict__ loss) {
+// Two waves per SIMD (256 registers, a 12 B spill): 0.263 instead of 0.281 ms per training step
+// of 66,048 samples on MI355X.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_nrc_train_fb(
+    GridInfo g0, const uint32_t* __restrict__ table_h, const _Float16* __restrict__ mlp_h,
+    const arc_path_sample* __restrict__ s, uint32_t n, uint32_t ld, _Float16* __restrict__ act,
+    float* __restrict__ table_grad, float2* __restrict__ enc_grad, float4* __restrict__ enc_pos,
+    float* __restrict__ loss) {
   __shared__ MlpLds L;
   __shared__ MlpLdsT T;
   __shared__ GridInfo sg;  // per-lane level lookups from LDS (as in k_nrc_infer)
@@ -418,9 +419,9 @@ __global__ __launch_bounds__(256) void k_nrc_train_fb(GridInfo g0, const uint32_
 // that way. Here workgroup (level l, chunk c) adds the gradients of its samples' 8 corners at
 // level l into an LDS copy of that level's table with LDS atomics, then stores the copy to its
 // partial slice, and k_nrc_reduce adds the slices into the table gradient: no global atomics.
-constexpr uint32_t kLdsScatterEntries = 1u << 14;  // 128 KB of FP32 pairs per workgroup
+constexpr uint32_t kLdsScatterEntries = 1u << 14;  // 128 KB of FP32 pairs per workgroup (of 1024 lanes: 16 waves per CU)
 
-__global__ __launch_bounds__(256) void k_nrc_scatter(GridInfo g0, const float2* __restrict__ enc_grad,
+__global__ __launch_bounds__(1024) void k_nrc_scatter(GridInfo g0, const float2* __restrict__ enc_grad,
                                                      const float4* __restrict__ enc_pos, uint32_t n, uint32_t ld,
                                                      uint32_t chunk, float* __restrict__ partial) {
   __shared__ float tab[2 * kLdsScatterEntries];
@@ -778,7 +779,7 @@ int arc_nrc_backward(arc_nrc* r, const arc_path_sample* d_samples, uint64_t n, v
     // chunks of at least 256 samples, at most one per partial slice
     const uint32_t parts = (uint32_t)std::min<uint64_t>(r->scatter_parts, std::max<uint64_t>(1, (n + 255) / 256));
     const uint32_t chunk = (uint32_t)((n + parts - 1) / parts);
-    hipLaunchKernelGGL(k_nrc_scatter, dim3(parts * (uint32_t)kLevels), dim3(256), 0, s, r->g, r->enc_grad, r->enc_pos,
+    hipLaunchKernelGGL(k_nrc_scatter, dim3(parts * (uint32_t)kLevels), dim3(1024), 0, s, r->g, r->enc_grad, r->enc_pos,
                        (uint32_t)n, r->act_ld, chunk, r->scatter_part);
     hipLaunchKernelGGL(k_nrc_reduce, dim3(2048), dim3(256), 0, s, reinterpret_cast<const float4*>(r->scatter_part),
                        parts, r->grid_params / 4, reinterpret_cast<float4*>(r->grid_g));
