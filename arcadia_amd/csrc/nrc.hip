@@ -65,17 +65,23 @@ __device__ __forceinline__ floatx4 mfma2(half4 a0, half4 a1, half4 b0, half4 b1,
   return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ uint32_t corner_index(const GridInfo& g, int l, uint32_t x, uint32_t y, uint32_t z) {
-  if (g.dense[l]) {
-    const uint32_t r = g.res1[l];
-    return (x + y * r + z * r * r) & g.table_mask;
-  }
-  return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & g.table_mask;
-}
-
 struct LevelPos {
   uint32_t c[3];
   float f[3];
+};
+
+// The 8 corner entries of a level around a position: vertex (x, y, z) of a dense level (res1^3 <=
+// table entries) is entry x + y res1 + z res1^2, of a hashed level x ^ y 2654435761 ^ z 805459861,
+// both masked to the table. The y and z terms take two values each, so they are formed once per
+// level and a corner costs one three-input add or XOR, a select and a mask (a branch per corner on
+// the dense flag diverged between the lane groups, which hold different levels).
+struct CornerIdx {
+  uint32_t x0, y0, y1, z0, z1, mask;
+  bool dense;
+  __device__ __forceinline__ uint32_t at(int k) const {
+    const uint32_t x = x0 + (uint32_t)(k & 1), y = (k & 2) ? y1 : y0, z = (k & 4) ? z1 : z0;
+    return (dense ? x + y + z : x ^ y ^ z) & mask;
+  }
 };
 
 __device__ __forceinline__ LevelPos level_pos(const GridInfo& g, int l, const float p[3]) {
@@ -89,6 +95,20 @@ __device__ __forceinline__ LevelPos level_pos(const GridInfo& g, int l, const fl
   return lp;
 }
 
+__device__ __forceinline__ CornerIdx corner_idx(const GridInfo& g, int l, const LevelPos& lp) {
+  CornerIdx c;
+  c.dense = g.dense[l] != 0u;
+  const uint32_t r = g.res1[l];
+  const uint32_t my = c.dense ? r : 2654435761u, mz = c.dense ? r * r : 805459861u;
+  c.x0 = lp.c[0];
+  c.y0 = lp.c[1] * my;
+  c.y1 = (lp.c[1] + 1u) * my;
+  c.z0 = lp.c[2] * mz;
+  c.z1 = (lp.c[2] + 1u) * mz;
+  c.mask = g.table_mask;
+  return c;
+}
+
 // Trilinear interpolation of the 2 features of level l at normalized position p. Tables hold at
 // most 16 x 2^24 entries (arc_nrc_create), so byte offsets into the FP16-pair table (4 B per
 // entry) and into its FP32 gradient (8 B per entry) fit 32 bits: the loads and atomics take a
@@ -96,6 +116,7 @@ __device__ __forceinline__ LevelPos level_pos(const GridInfo& g, int l, const fl
 __device__ __forceinline__ void encode_level(const GridInfo& g, const uint32_t* __restrict__ table_h, int l,
                                              const float p[3], float& e0, float& e1) {
   const LevelPos lp = level_pos(g, l, p);
+  const CornerIdx ci = corner_idx(g, l, lp);
   const uint32_t lbase = (uint32_t)l * (g.table_mask + 1u);
   e0 = 0.0f;
   e1 = 0.0f;
@@ -103,7 +124,7 @@ __device__ __forceinline__ void encode_level(const GridInfo& g, const uint32_t* 
     const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
     const float w = (dx ? lp.f[0] : 1.0f - lp.f[0]) * (dy ? lp.f[1] : 1.0f - lp.f[1]) * (dz ? lp.f[2] : 1.0f - lp.f[2]);
     const uint32_t v = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(table_h) +
-                                                          ((lbase + corner_index(g, l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz)) << 2));
+                                                          ((lbase + ci.at(k)) << 2));
     const float f0 = (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xFFFFu));
     const float f1 = (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
     e0 += w * f0;
@@ -115,11 +136,12 @@ __device__ __forceinline__ void scatter_level(const GridInfo& g, float* __restri
                                               float d0, float d1) {
   if (d0 == 0.0f && d1 == 0.0f) return;
   const LevelPos lp = level_pos(g, l, p);
+  const CornerIdx ci = corner_idx(g, l, lp);
   const uint32_t lbase = (uint32_t)l * (g.table_mask + 1u);
   for (int k = 0; k < 8; ++k) {
     const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
     const float w = (dx ? lp.f[0] : 1.0f - lp.f[0]) * (dy ? lp.f[1] : 1.0f - lp.f[1]) * (dz ? lp.f[2] : 1.0f - lp.f[2]);
-    const uint32_t idx = corner_index(g, l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz);
+    const uint32_t idx = ci.at(k);
     float* t = reinterpret_cast<float*>(reinterpret_cast<char*>(grad) + ((lbase + idx) << 3));
     atomicAdd(t, w * d0);
     atomicAdd(t + 1, w * d1);
@@ -439,10 +461,11 @@ __global__ __launch_bounds__(1024) void k_nrc_scatter(GridInfo g0, const float2*
     if (q.w == 0.0f || (d.x == 0.0f && d.y == 0.0f)) continue;
     const float p[3] = {q.x, q.y, q.z};
     const LevelPos lp = level_pos(g, (int)l, p);
+    const CornerIdx ci = corner_idx(g, (int)l, lp);
     for (int k = 0; k < 8; ++k) {
       const uint32_t dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
       const float w = (dx ? lp.f[0] : 1.0f - lp.f[0]) * (dy ? lp.f[1] : 1.0f - lp.f[1]) * (dz ? lp.f[2] : 1.0f - lp.f[2]);
-      const uint32_t e = corner_index(g, (int)l, lp.c[0] + dx, lp.c[1] + dy, lp.c[2] + dz);
+      const uint32_t e = ci.at(k);
       atomicAdd(&tab[2u * e], w * d.x);
       atomicAdd(&tab[2u * e + 1u], w * d.y);
     }
