@@ -189,30 +189,11 @@ def test_autotuner_keeps_results_bit_exact(ctx):
     assert len(W.diff_records(got, ref)) == 0
 
 
-def water_world(with_water: bool):
-    """The 2x2x2 cave world with a lake: a full 32x16x32 water block (its 16^3 cells collapse to
-    uniform water nodes), a thin water layer that leaves mixed water bricks, and water voxels
-    scattered through the caves."""
-    w = A.HostWorld((2, 2, 2))
-    p = A.gen_params(A.ARC_GEN_CAVES, seed=1, n_cuboids=120, height_min=96.0, height_max=352.0, wavelength=192.0)
-    w.generate(p)
-    if with_water:
-        z, y, x = np.mgrid[0:32, 0:16, 0:32]
-        lake = np.stack([x.ravel() + 160, y.ravel() + 240, z.ravel() + 160, np.full(x.size, 18)], 1)
-        z, x = np.mgrid[0:48, 0:48]
-        layer = np.stack([x.ravel() + 100, np.full(x.size, 301), z.ravel() + 300, np.full(x.size, 18)], 1)
-        rng = np.random.default_rng(4)
-        drops = np.concatenate([rng.integers(0, 512, (4000, 3)), np.full((4000, 1), 18)], 1)
-        w.set_voxels(np.concatenate([lake, layer, drops]).astype(np.int32))
-    w.pack()
-    return w
-
-
 @pytest.mark.parametrize("ignore_water", [True, False])
 def test_water_world(ctx, ignore_water):
     """The ignore_water rules (uniform water nodes stepped over whole, water voxels stepped through,
     traversal64.wgsl:398-417, 451-465) on a world that holds water, all three ray classes."""
-    pools = water_world(True).pools()
+    pools = W.water_world(True).pools()
     assert (np.asarray(pools[3]).view(np.uint8) == 18).any()
     rays = W.random_rays(60000, (2, 2, 2), seed=41)
     for bias in (0, 1, 2):
@@ -223,7 +204,7 @@ def test_water_world(ctx, ignore_water):
 def test_water_patched_into_dry_world(ctx):
     """A world without water traces with ignore_water set (the pools cannot hold water, so the
     kernels skip its tests); water patched in afterwards must be honoured at once."""
-    w = water_world(False)
+    w = W.water_world(False)
     ctx.upload(*w.pools(copy=False))
     w.clear_patches()
     params = A.TraceParams.exact(2, ignore_water=True)

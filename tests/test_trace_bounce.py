@@ -116,3 +116,40 @@ def test_bounce_launch_random_rays_and_seeds(ctx):
     for seed in (1, 2**40 + 3):
         check(f"random rays, seed {seed}", run(ctx, p0, pb, rays, seed=seed),
               expected(pools, p0, pb, rays, seed=seed))
+
+
+def test_bounce_launch_water_world(ctx):
+    """A world that holds water (the kernel's water-test instance): primary rays that stop at water
+    and bounce rays that step through it (ignore_water), from inside and around the world."""
+    pools = W.water_world(True).pools()
+    assert (np.asarray(pools[3]).view(np.uint8) == 18).any()
+    ctx.upload(*pools)
+    rays = W.random_rays(30000, (2, 2, 2), seed=43)
+    for p0, pb in ((A.TraceParams.exact(0), A.TraceParams.exact(2, ignore_water=True)),
+                   (A.TraceParams.exact(0, ignore_water=True), A.TraceParams.exact(2, ignore_water=False))):
+        check(f"water world, primary ignore_water {p0.ignore_water}, bounce {pb.ignore_water}",
+              run(ctx, p0, pb, rays), expected(pools, p0, pb, rays))
+
+
+def test_bounce_autotune(ctx):
+    """arc_autotune_bounce picks a shape for this ray set and installs it: the next launch uses it
+    and its records are exact."""
+    import torch
+
+    pools = W.caves_world(1)
+    ctx.upload(*pools)
+    rays = caves_rays(320, 180)
+    p0, pb = A.TraceParams.exact(0), A.TraceParams.exact(2, ignore_water=True)
+    n = len(rays)
+    d_r = torch.from_numpy(np.ascontiguousarray(rays).view(np.uint8)).cuda()
+    d_h = torch.zeros(2 * n * 16, dtype=torch.uint8, device="cuda")
+    saved = ctx.launch_config(A.ARC_LAUNCH_CLASS_BOUNCE)
+    try:
+        cfg, ms = ctx.autotune_bounce(p0, pb, d_r.data_ptr(), n, SEED, BOUNCE_M, d_h.data_ptr())
+        print(f"[bounce] autotuned {cfg} {ms:.4f} ms", flush=True)
+        assert ms > 0.0 and cfg["block_threads"] in (128, 256) and cfg["refill_lanes"] > 0 and cfg["batch_rays"] > 0
+        assert ctx.launch_config(A.ARC_LAUNCH_CLASS_BOUNCE) == cfg
+        check("after autotune", run(ctx, p0, pb, rays), expected(pools, p0, pb, rays))
+    finally:
+        ctx.set_launch_config_class(A.ARC_LAUNCH_CLASS_BOUNCE, *[saved[k] for k in
+                                    ("blocks_per_cu", "block_threads", "refill_lanes", "batch_rays")])

@@ -83,3 +83,22 @@ def diff_records(a: np.ndarray, b: np.ndarray):
     wa = a.view(np.uint32).reshape(len(a), -1)
     wb = b.view(np.uint32).reshape(len(b), -1)
     return np.nonzero((wa != wb).any(1))[0]
+
+
+def water_world(with_water: bool):
+    """The 2x2x2 cave world with a lake: a full 32x16x32 water block (its 16^3 cells collapse to
+    uniform water nodes), a thin water layer that leaves mixed water bricks, and water voxels
+    scattered through the caves."""
+    w = A.HostWorld((2, 2, 2))
+    p = A.gen_params(A.ARC_GEN_CAVES, seed=1, n_cuboids=120, height_min=96.0, height_max=352.0, wavelength=192.0)
+    w.generate(p)
+    if with_water:
+        z, y, x = np.mgrid[0:32, 0:16, 0:32]
+        lake = np.stack([x.ravel() + 160, y.ravel() + 240, z.ravel() + 160, np.full(x.size, 18)], 1)
+        z, x = np.mgrid[0:48, 0:48]
+        layer = np.stack([x.ravel() + 100, np.full(x.size, 301), z.ravel() + 300, np.full(x.size, 18)], 1)
+        rng = np.random.default_rng(4)
+        drops = np.concatenate([rng.integers(0, 512, (4000, 3)), np.full((4000, 1), 18)], 1)
+        w.set_voxels(np.concatenate([lake, layer, drops]).astype(np.int32))
+    w.pack()
+    return w
