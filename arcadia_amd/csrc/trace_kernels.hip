@@ -272,8 +272,9 @@ __device__ __forceinline__ void path_start(const DevWorld& w, const DevParams& p
 }
 
 // Writes the finished ray's record, then starts its bounce or closes the slot.
+// `frames`: arcmath::bounce_frame of the nine face codes (the workgroup's LDS table).
 __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& pb, const PathCtl& pc, uint4* out,
-                                            Lane& L, float& tmax, uint4* C) {
+                                            Lane& L, float& tmax, uint4* C, const arcmath::BounceFrame* frames) {
   const uint32_t depth = fget(L.st, F_DEPTH, 3);
   const uint32_t steps = L.step > 0xFFFFu ? 0xFFFFu : L.step;
   const bool hit = fget(L.st, F_HIT, 1) != 0u;
@@ -299,7 +300,8 @@ __device__ __forceinline__ void path_finish(const DevWorld& w, const DevParams& 
   if (hit && depth < pc.n_bounces) {
     const float ov[3] = {L.ox, L.oy, L.oz}, dr[3] = {L.dx, L.dy, L.dz};
     arc_ray r;
-    arcmath::bounce_ray(ov, dr, exact_t, face, pc.seed, (uint64_t)depth * pc.n_items + item, pc.bounce_tmax, &r);
+    arcmath::bounce_ray_framed(ov, dr, exact_t, face, frames[face], pc.seed, (uint64_t)depth * pc.n_items + item,
+                               pc.bounce_tmax, &r);
     ray_init<ARC_SEM_WORLD>(w, pb, r, L, C);
     L.st = fput(L.st, F_DEPTH, 3, depth + 1u);
     tmax = pc.bounce_tmax;
@@ -323,7 +325,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
     uint32_t refill, uint32_t batch) {
   __shared__ uint4 s_cold[kColdSlots * BLOCK];
   __shared__ RootLdsT<NCH> s_roots;
+  __shared__ arcmath::BounceFrame s_frame[9];  // the bounce lobe's frame per face code
   uint4* const C = s_cold + threadIdx.x;
+  if (threadIdx.x < 9u) s_frame[threadIdx.x] = arcmath::bounce_frame(threadIdx.x);
+  __syncthreads();
   const bool lds_roots = stage_roots(w, s_roots);
   const uint32_t lane = __lane_id();
   Lane L;
@@ -359,7 +364,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) v
       const uint64_t pending = __ballot(idle_lane && fget(L.st, F_HASRAY, 1) != 0u);
       // drained, nothing pending, every lane idle: done (uniform exit)
       if (rfl((uint32_t)(drained && nidle == 64u && pending == 0ull))) break;
-      if (idle_lane && fget(L.st, F_HASRAY, 1)) path_finish(w, pb, pc, out, L, tmax, C);
+      if (idle_lane && fget(L.st, F_HASRAY, 1)) path_finish(w, pb, pc, out, L, tmax, C, s_frame);
       if (!drained && nidle >= refill) {
         const bool needs = fget(L.st, F_LIVE, 1) == 0u && fget(L.st, F_HASRAY, 1) == 0u;
         const uint64_t nm = __ballot(needs);
