@@ -279,7 +279,8 @@ __global__ __launch_bounds__(256) void k_sharc_query_paths(const arc_path_sample
 
 // Evicts every surfel whose cell overlaps one of the spheres (x, y, z, r in meters, widened by
 // the 2 cm anchor offset): one workgroup per (sphere, cascade level); each thread walks cells of
-// the sphere's bounding box at that level and probes the 6 face keys of each.
+// the sphere's bounding box at that level and probes the 6 face keys of each. Most probed buckets
+// hold none of the keys, so the keys are read first and the atomic runs only on a match.
 __global__ __launch_bounds__(256) void k_sharc_invalidate(const float4* __restrict__ spheres, uint32_t bucket_mask,
                                                           uint32_t* __restrict__ keys, uint4* __restrict__ accum,
                                                           uint4* __restrict__ resolved) {
@@ -301,9 +302,13 @@ __global__ __launch_bounds__(256) void k_sharc_invalidate(const float4* __restri
       const uint32_t key = pcg(bucket ^ (cx * 1013904223u) ^ (cy * 1664525u) ^ (cz * 2891336453u) ^ (nid << 20u) ^
                                (lod << 24u)) |
                            kKeyValid;
+      // the bucket's four keys in one load; a compare-and-swap only where one matches (no thread of
+      // this launch writes a valid key, so a slot that does not hold `key` now will not later)
+      const uint4 held = *reinterpret_cast<const uint4*>(keys + bucket * kWays);
+      const uint32_t hk[kWays] = {held.x, held.y, held.z, held.w};
       for (uint32_t w = 0; w < kWays; ++w) {
         const uint32_t slot = bucket * kWays + w;
-        if (atomicCAS(keys + slot, key, 0u) == key) {
+        if (hk[w] == key && atomicCAS(keys + slot, key, 0u) == key) {
           accum[slot] = make_uint4(0u, 0u, 0u, 0u);
           resolved[slot] = make_uint4(0u, 0u, 0u, 0u);
         }
