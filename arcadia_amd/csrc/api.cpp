@@ -7,9 +7,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <new>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/arcadia_hip.h"
@@ -124,9 +126,13 @@ bool valid_params(const arc_trace_params* p) {
 // wave refills) and window size per queue claim.
 namespace {
 
-template <typename Run>
+// Launch-shape autotuner shared by the traversal classes, the one-launch frame and the batched
+// views: `run` launches with the shape in *slot; threads x blocks x refill x batch candidates,
+// `skip(threads, blocks)` leaves out shapes that cannot launch.
+template <typename Run, typename Skip>
 int tune_shapes(arc_ctx* c, arc_launch_config* slot, void* stream, uint32_t reps, Run run,
-                       arc_launch_config* best, float* best_ms) {
+                arc_launch_config* best, float* best_ms, std::initializer_list<uint32_t> kThreads,
+                std::initializer_list<uint32_t> kBlocks, Skip skip) {
   if (reps == 0) reps = 3;
   if (reps > 64) reps = 64;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -135,37 +141,58 @@ int tune_shapes(arc_ctx* c, arc_launch_config* slot, void* stream, uint32_t reps
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
   HIP_TRY(c, hipEventCreate(&e1));
-  static const uint32_t kThreads[] = {128, 256};
-  static const uint32_t kBlocks[] = {0, 4, 6};
   static const uint32_t kRefill[] = {8, 16, 24, 32, 48};
   static const uint32_t kBatch[] = {32, 64, 128};
-  arc_launch_config win = saved;
-  float win_ms = 1e30f;
   int rc = ARC_OK;
-  std::vector<float> t(reps);
+  // median of `r` timed launches (after one warm-up) of the shape in *slot
+  auto time_shape = [&](uint32_t r, std::vector<float>& t) -> float {
+    rc = run();  // warm-up
+    for (uint32_t i = 0; i < r && rc == ARC_OK; ++i) {
+      if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
+      if (rc == ARC_OK) rc = run();
+      if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
+      if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
+      if (rc == ARC_OK && hipEventElapsedTime(&t[i], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
+    }
+    if (rc != ARC_OK) return 1e30f;
+    std::vector<float> sorted(t.begin(), t.begin() + r);
+    std::sort(sorted.begin(), sorted.end());
+    return sorted[r / 2];
+  };
+  // Sweep every shape with `reps` launches, then re-time the four fastest with four times as many
+  // (at least 32): neighbouring shapes differ by about the launch-to-launch noise, so the short
+  // sweep alone can pick a shape 1-2% slower than the best.
+  const uint32_t reps2 = std::min(64u, std::max(reps, 8u) * 4u);
+  std::vector<std::pair<float, arc_launch_config>> swept;
+  std::vector<float> t(reps2);
   for (uint32_t bt : kThreads)
     for (uint32_t bpc : kBlocks)
       for (uint32_t rf : kRefill)
         for (uint32_t bat : kBatch) {
           if (rc != ARC_OK) break;
+          if (skip(bt, bpc)) continue;
           *slot = arc_launch_config{bpc, bt, rf, bat};
-          rc = run();  // warm-up
-          for (uint32_t r = 0; r < reps && rc == ARC_OK; ++r) {
-            if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK) rc = run();
-            if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
-          }
+          const float med = time_shape(reps, t);
           if (rc != ARC_OK) break;
-          std::vector<float> sorted = t;
-          std::sort(sorted.begin(), sorted.end());
-          const float med = sorted[reps / 2];
-          if (med < win_ms) {
-            win_ms = med;
-            win = *slot;
-          }
+          swept.emplace_back(med, *slot);
         }
+  arc_launch_config win = saved;
+  float win_ms = 1e30f;
+  if (rc == ARC_OK) {
+    std::sort(swept.begin(), swept.end(),
+              [](const std::pair<float, arc_launch_config>& x, const std::pair<float, arc_launch_config>& y) {
+                return x.first < y.first;
+              });
+    for (size_t k = 0; k < swept.size() && k < 4; ++k) {
+      *slot = swept[k].second;
+      const float med = time_shape(reps2, t);
+      if (rc != ARC_OK) break;
+      if (med < win_ms) {
+        win_ms = med;
+        win = *slot;
+      }
+    }
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (rc != ARC_OK) {
@@ -799,7 +826,7 @@ int arc_autotune(arc_ctx* c, const arc_trace_params* params, const arc_ray* d_ra
   if (n == 0 || !d_rays || !d_hits) return fail(c, ARC_ERR_INVALID, "autotune needs a non-empty ray set");
   return tune_shapes(
       c, &c->cfg[lod_class(params)], stream, reps, [&] { return launch(c, params, d_rays, n, d_hits, false, stream); },
-      best, best_ms);
+      best, best_ms, {128u, 256u}, {0u, 4u, 6u}, [](uint32_t, uint32_t) { return false; });
 }
 
 // ---- primary rays and their bounces in one launch ------------------------------------------
@@ -868,7 +895,7 @@ int arc_autotune_bounce(arc_ctx* c, const arc_trace_params* primary, const arc_t
   return tune_shapes(
       c, &c->cfg_bounce, stream, reps,
       [&] { return arc_trace_bounce_device(c, primary, bounce, d_rays, n, seed, bounce_max_dist_m, d_hits, stream); },
-      best, best_ms);
+      best, best_ms, {128u, 256u}, {0u, 4u, 6u}, [](uint32_t, uint32_t) { return false; });
 }
 
 int arc_primary_gbuffer(arc_ctx* c, const arc_trace_params* params, const arc_gbuffer_params* g, uint32_t* d_depth_mat,
@@ -1137,53 +1164,12 @@ int arc_batch_autotune(arc_batch* b, arc_hit* d_hits, void* stream, uint32_t rep
   if (!b) return ARC_ERR_INVALID;
   arc_ctx* c = b->ctx;
   if (!d_hits) return fail(c, ARC_ERR_INVALID, "null hit buffer");
-  if (reps == 0) reps = 3;
-  if (reps > 64) reps = 64;
-  HIP_TRY(c, hipSetDevice(c->device));
-  hipStream_t s = (hipStream_t)stream;
-  hipEvent_t e0, e1;
-  HIP_TRY(c, hipEventCreate(&e0));
-  HIP_TRY(c, hipEventCreate(&e1));
-  // 1024 = one workgroup per CU with the chunk roots of up to 4096 chunks in LDS
-  static const uint32_t kThreads[] = {128, 256, arcdev::kPathLargeBlock};
-  static const uint32_t kBlocks[] = {0, 3, 4};
-  static const uint32_t kRefill[] = {8, 16, 24, 32, 48};
-  static const uint32_t kBatch[] = {32, 64, 128};
-  const arc_launch_config saved = b->cfg;
-  arc_launch_config win = saved;
-  float win_ms = 1e30f;
-  int rc = ARC_OK;
-  std::vector<float> t(reps);
-  for (uint32_t bt : kThreads)
-    for (uint32_t bpc : kBlocks)
-      for (uint32_t rf : kRefill)
-        for (uint32_t bat : kBatch) {
-          if (rc != ARC_OK) break;
-          if (bt == arcdev::kPathLargeBlock && bpc != 0) continue;  // its LDS fits one workgroup per CU
-          b->cfg = arc_launch_config{bpc, bt, rf, bat};
-          rc = batch_launch(b, d_hits, s);
-          for (uint32_t r = 0; r < reps && rc == ARC_OK; ++r) {
-            if (hipEventRecord(e0, s) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK) rc = batch_launch(b, d_hits, s);
-            if (rc == ARC_OK && hipEventRecord(e1, s) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK && hipEventSynchronize(e1) != hipSuccess) rc = ARC_ERR_HIP;
-            if (rc == ARC_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = ARC_ERR_HIP;
-          }
-          if (rc != ARC_OK) break;
-          std::vector<float> sorted = t;
-          std::sort(sorted.begin(), sorted.end());
-          if (sorted[reps / 2] < win_ms) {
-            win_ms = sorted[reps / 2];
-            win = b->cfg;
-          }
-        }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  b->cfg = rc == ARC_OK ? win : saved;
-  if (rc != ARC_OK) return rc;
-  if (best) *best = win;
-  if (best_ms) *best_ms = win_ms;
-  return ARC_OK;
+  // 1024 = one workgroup per CU with the chunk roots of up to 4096 chunks in LDS; its LDS fits one
+  // workgroup per CU, so it is tried at the occupancy limit only
+  return tune_shapes(
+      c, &b->cfg, stream, reps, [&] { return batch_launch(b, d_hits, (hipStream_t)stream); }, best, best_ms,
+      {128u, 256u, arcdev::kPathLargeBlock}, {0u, 3u, 4u},
+      [](uint32_t bt, uint32_t bpc) { return bt == arcdev::kPathLargeBlock && bpc != 0u; });
 }
 
 int arc_batch_shade(arc_batch* b, const arc_hit* d_hits, const arc_shade_params* sp, arc_path_sample* d_samples,
