@@ -482,8 +482,10 @@ int arc_set_launch_config_class(arc_ctx* ctx, uint32_t lod_class, const arc_laun
 int arc_get_launch_config_class(const arc_ctx* ctx, uint32_t lod_class, arc_launch_config* cfg);
 /* Autotuner (no reference counterpart; the reference dispatches fixed 8x8 workgroups): times
  * candidate launch shapes of the persistent kernel on this ray set with HIP events on
- * `stream`, keeps the fastest for the ray class of `params` and returns it (and its time in
- * ms, median of `reps` launches). d_hits receives the last traced results. Blocking. */
+ * `stream` (every shape with `reps` launches, then the four fastest again with max(reps, 8) * 4
+ * launches, at most 64), keeps the fastest for the ray class of `params` and returns it (and
+ * its time in ms, the median of the second pass). d_hits receives the last traced results.
+ * Blocking. */
 int arc_autotune(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_rays, uint64_t n,
                  arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
                  float* best_ms);
@@ -526,7 +528,8 @@ int arc_batch_trace(arc_batch* b, arc_hit* d_hits, void* stream);
  * growth moves the pools: replay then fails until the batch is captured again. */
 int arc_batch_capture(arc_batch* b, arc_hit* d_hits);
 int arc_batch_replay(arc_batch* b, void* stream);
-/* Launch shape of the path kernel, and its autotuner (same candidate grid as arc_autotune). */
+/* Launch shape of the path kernel, and its autotuner (arc_autotune's search, plus 1024-lane
+ * workgroups at the occupancy limit). */
 int arc_batch_set_launch_config(arc_batch* b, const arc_launch_config* cfg);
 int arc_batch_get_launch_config(const arc_batch* b, arc_launch_config* cfg);
 int arc_batch_autotune(arc_batch* b, arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
