@@ -153,3 +153,28 @@ def test_bounce_autotune(ctx):
     finally:
         ctx.set_launch_config_class(A.ARC_LAUNCH_CLASS_BOUNCE, *[saved[k] for k in
                                     ("blocks_per_cu", "block_threads", "refill_lanes", "batch_rays")])
+
+
+def test_bounce_launch_repeats_are_identical(ctx):
+    """Launches repeated across launch shapes and seeds: every launch of a seed gives the records of
+    its first launch (the message words and queue counters return to zero between launches, and no
+    result depends on the order in which waves publish or take messages)."""
+    pools = W.caves_world(1)
+    ctx.upload(*pools)
+    rays = caves_rays(480, 270)
+    p0, pb = A.TraceParams.exact(0), A.TraceParams.exact(2, ignore_water=True)
+    saved = ctx.launch_config(A.ARC_LAUNCH_CLASS_BOUNCE)
+    first = {}
+    try:
+        for k, shape in enumerate([(0, 256, 24, 64), (1, 64, 1, 1), (4, 256, 8, 32), (0, 128, 48, 128), (6, 256, 64, 256)] * 3):
+            ctx.set_launch_config_class(A.ARC_LAUNCH_CLASS_BOUNCE, *shape)
+            for seed in (SEED, 2**40 + 3):
+                got = run(ctx, p0, pb, rays, seed=seed)
+                if seed not in first:
+                    first[seed] = got
+                    check(f"seed {seed}, first launch", got, expected(pools, p0, pb, rays, seed=seed))
+                else:
+                    assert np.array_equal(got.view(np.uint32), first[seed].view(np.uint32)), (k, shape, seed)
+    finally:
+        ctx.set_launch_config_class(A.ARC_LAUNCH_CLASS_BOUNCE, *[saved[k] for k in
+                                    ("blocks_per_cu", "block_threads", "refill_lanes", "batch_rays")])
