@@ -48,6 +48,7 @@ struct arc_ctx {
   uint64_t scratch_n = 0;
   arc_launch_config cfg[3]{};  // per ray class (lod_bias 0, 1, >=2)
   arc_launch_config cfg_bounce{};  // arc_trace_bounce_device
+  arc_launch_config cfg_shadow{};  // arc_trace_shadow_device
   // arc_trace_bounce_device: one message word per primary ray (zero between launches)
   uint64_t* d_msg = nullptr;
   uint64_t msg_n = 0;
@@ -222,6 +223,7 @@ int arc_create(int device, arc_ctx** out) {
   c->cfg_bounce.blocks_per_cu = 0;
   c->cfg_bounce.refill_lanes = 24;
   c->cfg_bounce.batch_rays = 64;
+  c->cfg_shadow = default_config();
   if (hipSetDevice(device) != hipSuccess) {
     delete c;
     return ARC_ERR_HIP;
@@ -322,14 +324,24 @@ int arc_get_launch_config(const arc_ctx* c, arc_launch_config* cfg) {
   return ARC_OK;
 }
 
+// Launch-shape slot of a class: the three LOD classes of the traversal, the one-launch frame, the
+// shadow walkers.
+static arc_launch_config* class_slot(arc_ctx* c, uint32_t cls) {
+  if (cls == ARC_LAUNCH_CLASS_BOUNCE) return &c->cfg_bounce;
+  if (cls == ARC_LAUNCH_CLASS_SHADOW) return &c->cfg_shadow;
+  return cls < ARC_LAUNCH_CLASS_BOUNCE ? &c->cfg[cls] : nullptr;
+}
+
 int arc_set_launch_config_class(arc_ctx* c, uint32_t cls, const arc_launch_config* cfg) {
-  if (!c || !cfg || cls > ARC_LAUNCH_CLASS_BOUNCE) return ARC_ERR_INVALID;
-  return normalize_config(c, cfg, cls == ARC_LAUNCH_CLASS_BOUNCE ? &c->cfg_bounce : &c->cfg[cls]);
+  if (!c || !cfg || !class_slot(c, cls)) return ARC_ERR_INVALID;
+  return normalize_config(c, cfg, class_slot(c, cls));
 }
 
 int arc_get_launch_config_class(const arc_ctx* c, uint32_t cls, arc_launch_config* cfg) {
-  if (!c || !cfg || cls > ARC_LAUNCH_CLASS_BOUNCE) return ARC_ERR_INVALID;
-  *cfg = cls == ARC_LAUNCH_CLASS_BOUNCE ? c->cfg_bounce : c->cfg[cls];
+  if (!c || !cfg) return ARC_ERR_INVALID;
+  const arc_launch_config* slot = class_slot(const_cast<arc_ctx*>(c), cls);
+  if (!slot) return ARC_ERR_INVALID;
+  *cfg = *slot;
   return ARC_OK;
 }
 
@@ -707,9 +719,9 @@ int arc_trace_shadow_device(arc_ctx* c, const arc_shadow_params* sp, const arc_r
   a.rec = d_rec;
   a.n = n;
   a.queue = c->d_queue + 2u * arcdev::kQueueRegionWords;  // the walkers' own queue
-  // The walkers keep the default shape: the per-class traversal shapes (and the autotuner, which
-  // sets them for long rays) measured slower on these short rays (sun 0.220 vs 0.199 ms).
-  const arc_launch_config cfg = default_config();
+  // The walkers have their own shape (ARC_LAUNCH_CLASS_SHADOW, arc_autotune_shadow): the traversal
+  // classes' shapes, tuned for long rays, measured slower on these short rays (sun 0.220 vs 0.199 ms).
+  const arc_launch_config cfg = c->cfg_shadow;
   uint32_t bpc = cfg.blocks_per_cu;
   if (bpc == 0) {
     hipError_t oe = arcdev::shadow_occupancy(cfg.block_threads, &bpc);
@@ -724,6 +736,15 @@ int arc_trace_shadow_device(arc_ctx* c, const arc_shadow_params* sp, const arc_r
   hipError_t e = arcdev::launch_shadow(a, (hipStream_t)stream);
   if (e != hipSuccess) return fail(c, ARC_ERR_HIP, "shadow launch: %s", hipGetErrorString(e));
   return ARC_OK;
+}
+
+int arc_autotune_shadow(arc_ctx* c, const arc_shadow_params* sp, const arc_ray* d_rays, uint64_t n, float* d_vis,
+                        arc_shadow_rec* d_rec, void* stream, uint32_t reps, arc_launch_config* best, float* best_ms) {
+  if (!c) return ARC_ERR_INVALID;
+  if (!sp || n == 0 || !d_rays || !d_vis) return fail(c, ARC_ERR_INVALID, "autotune needs a non-empty ray set");
+  return tune_shapes(
+      c, &c->cfg_shadow, stream, reps, [&] { return arc_trace_shadow_device(c, sp, d_rays, n, d_vis, d_rec, stream); },
+      best, best_ms, {128u, 256u}, {0u, 4u, 6u}, [](uint32_t, uint32_t) { return false; });
 }
 
 int arc_trace_rigid_device(arc_ctx* c, const arc_rigid_body* bodies, uint32_t n_bodies, const arc_ray* d_rays,

@@ -24,6 +24,7 @@ ARC_ERR_CORRUPT = -6
 ARC_SEM_WORLD = 0
 ARC_SEM_CHUNK = 1
 ARC_LAUNCH_CLASS_BOUNCE = 3  # arc_trace_bounce_device launch shape slot
+ARC_LAUNCH_CLASS_SHADOW = 4  # arc_trace_shadow_device launch shape slot
 ARC_GEN_HEIGHTMAP = 0
 ARC_GEN_CAVES = 1
 ARC_GEN_SPARSE = 2
@@ -322,6 +323,9 @@ _SIGS = [
                                      C.POINTER(LaunchConfig), C.POINTER(C.c_float)]),
     ("arc_autotune", C.c_int, [C.c_void_p, C.POINTER(TraceParams), C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
                                C.c_uint32, C.POINTER(LaunchConfig), C.POINTER(C.c_float)]),
+    ("arc_autotune_shadow", C.c_int, [C.c_void_p, C.POINTER(ShadowParams), C.c_void_p, C.c_uint64, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(LaunchConfig),
+                                      C.POINTER(C.c_float)]),
     ("arc_world_host_create", C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     ("arc_world_host_destroy", None, [C.c_void_p]),
     ("arc_world_host_set_chunk_dense", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
@@ -914,6 +918,15 @@ class Context:
         sp = ShadowParams(mode, max_steps)
         self._check(lib().arc_trace_shadow_device(self._h, C.byref(sp), d_rays, n, d_vis, d_rec or None,
                                                   stream or None), "arc_trace_shadow_device")
+
+    def autotune_shadow(self, mode: int, max_steps: int, d_rays: int, n: int, d_vis: int, d_rec: int = 0,
+                        stream: int = 0, reps: int = 3):
+        """Native autotuner for the shadow walkers' launch shape (class ARC_LAUNCH_CLASS_SHADOW) on this ray set."""
+        sp = ShadowParams(mode, max_steps)
+        cfg, ms = LaunchConfig(), C.c_float()
+        self._check(lib().arc_autotune_shadow(self._h, C.byref(sp), d_rays, n, d_vis, d_rec or None, stream or None,
+                                              reps, C.byref(cfg), C.byref(ms)), "arc_autotune_shadow")
+        return {k: getattr(cfg, k) for k, _ in LaunchConfig._fields_}, ms.value
 
     def trace_rigid_device(self, bodies: np.ndarray, d_rays: int, n: int, d_out: int, stream: int = 0) -> None:
         """trace_rigid_bodies over device rays; `bodies` is a host RIGID_BODY_DTYPE array (<= 32 used),

@@ -354,6 +354,10 @@ def shadow_leg(args, ctx, pools, prays, phits, stream, threads):
         d_r = torch.from_numpy(rays.view(np.uint8)).cuda()
         d_v = torch.empty(n, dtype=torch.float32, device="cuda")
         d_rec = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+        if not args.no_autotune:  # the walkers' launch shape, tuned on this mode's rays (untimed setup)
+            cfg, tms = ctx.autotune_shadow(mode, 72, d_r.data_ptr(), n, d_v.data_ptr(), d_rec.data_ptr(), sptr)
+            out.setdefault("launch", {})[name] = cfg
+            print(f"[bench] shadow {name} autotuned: {cfg} ({tms:.3f} ms)", file=sys.stderr, flush=True)
         for _ in range(2):
             ctx.trace_shadow_device(mode, 72, d_r.data_ptr(), n, d_v.data_ptr(), d_rec.data_ptr(), sptr)
         k = max(3, min(args.steps, 10))

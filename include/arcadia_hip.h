@@ -269,8 +269,9 @@ typedef struct arc_shadow_rec {
   uint8_t reserved;
 } arc_shadow_rec;
 /* Device pointers, asynchronous on `stream`: d_vis[i] = visibility of ray i (0.0 / 1.0);
- * d_rec may be NULL. The walkers launch with the default shape (6 workgroups of 256 per CU,
- * refill 16, windows of 64 rays), whatever arc_set_launch_config / arc_autotune set for traces. */
+ * d_rec may be NULL. The walkers launch with their own shape, launch class
+ * ARC_LAUNCH_CLASS_SHADOW (default: 6 workgroups of 256 per CU, refill 16, windows of 64 rays;
+ * arc_autotune_shadow tunes it), whatever arc_set_launch_config / arc_autotune set for traces. */
 int arc_trace_shadow_device(arc_ctx* ctx, const arc_shadow_params* params, const arc_ray* d_rays, uint64_t n,
                             float* d_vis, arc_shadow_rec* d_rec, void* stream);
 
@@ -478,6 +479,8 @@ int arc_get_launch_config(const arc_ctx* ctx, arc_launch_config* cfg);
 /* Launch shape of one ray class (lod_class 0, 1, 2 = the lod_bias classes of the params), or of
  * arc_trace_bounce_device's launch (ARC_LAUNCH_CLASS_BOUNCE). */
 #define ARC_LAUNCH_CLASS_BOUNCE 3u
+/* arc_trace_shadow_device's launch (both walker modes). */
+#define ARC_LAUNCH_CLASS_SHADOW 4u
 int arc_set_launch_config_class(arc_ctx* ctx, uint32_t lod_class, const arc_launch_config* cfg);
 int arc_get_launch_config_class(const arc_ctx* ctx, uint32_t lod_class, arc_launch_config* cfg);
 /* Autotuner (no reference counterpart; the reference dispatches fixed 8x8 workgroups): times
@@ -493,6 +496,11 @@ int arc_autotune(arc_ctx* ctx, const arc_trace_params* params, const arc_ray* d_
 int arc_autotune_bounce(arc_ctx* ctx, const arc_trace_params* primary, const arc_trace_params* bounce,
                         const arc_ray* d_rays, uint64_t n, uint64_t seed, float bounce_max_dist_m,
                         arc_hit* d_hits, void* stream, uint32_t reps, arc_launch_config* best,
+                        float* best_ms);
+/* The same search for arc_trace_shadow_device's launch on this ray set (class
+ * ARC_LAUNCH_CLASS_SHADOW; d_vis / d_rec receive the last launch's results). */
+int arc_autotune_shadow(arc_ctx* ctx, const arc_shadow_params* sp, const arc_ray* d_rays, uint64_t n, float* d_vis,
+                        arc_shadow_rec* d_rec, void* stream, uint32_t reps, arc_launch_config* best,
                         float* best_ms);
 
 /* ---- batched views (one persistent launch) ----------------------------------------------- */

@@ -231,3 +231,33 @@ def test_gpu_walkers_match_oracle_on_caves_world(ctx, mode):
         assert same_records(rec, orec), (mode, steps)
         outcomes = np.bincount(orec["outcome"], minlength=6)
         assert outcomes[A.ARC_SHADOW_BLOCKED] > 1000 and outcomes[A.ARC_SHADOW_CLEAR] > 1000, outcomes
+
+
+@pytest.mark.gpu
+def test_gpu_walker_autotune_and_shapes_keep_records(ctx):
+    """The walkers' launch class (ARC_LAUNCH_CLASS_SHADOW): arc_autotune_shadow installs a shape, and
+    records under it and under extreme shapes equal the oracle's."""
+    import torch
+
+    pools = W.caves_world(1, (2, 2, 2), 120)
+    ctx.upload(*pools)
+    rays = W.random_rays(30000, (2, 2, 2), seed=5, inside=True, max_dist=40.0)
+    saved = ctx.launch_config(A.ARC_LAUNCH_CLASS_SHADOW)
+    try:
+        for mode in (A.ARC_SHADOW_SUN, A.ARC_SHADOW_DI):
+            ovis, orec = shadow(pools, mode, 72, rays)
+            n = len(rays)
+            d_r = torch.from_numpy(np.ascontiguousarray(rays).view(np.uint8)).cuda()
+            d_v = torch.empty(n, dtype=torch.float32, device="cuda")
+            d_rec = torch.empty(n * 16, dtype=torch.uint8, device="cuda")
+            cfg, ms = ctx.autotune_shadow(mode, 72, d_r.data_ptr(), n, d_v.data_ptr(), d_rec.data_ptr())
+            assert ms > 0.0 and ctx.launch_config(A.ARC_LAUNCH_CLASS_SHADOW) == cfg
+            for shape in (None, (1, 64, 1, 1), (0, 128, 48, 256)):
+                if shape is not None:
+                    ctx.set_launch_config_class(A.ARC_LAUNCH_CLASS_SHADOW, *shape)
+                vis, rec = gpu_shadow(ctx, mode, 72, rays)
+                assert np.array_equal(vis, ovis), (mode, shape)
+                assert same_records(rec, orec), (mode, shape)
+    finally:
+        ctx.set_launch_config_class(A.ARC_LAUNCH_CLASS_SHADOW, *[saved[k] for k in
+                                    ("blocks_per_cu", "block_threads", "refill_lanes", "batch_rays")])
